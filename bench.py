@@ -1,0 +1,212 @@
+#!/usr/bin/env python
+"""Benchmark of the DIT/LDM spectral-synthesis path on one MI355X.
+
+Default workload (``co2_5m``): 5 M synthetic CO2-like lines, 2000-2500 cm-1
+at wstep 0.001 cm-1 (500,002 grid points), T=1500 K, p=1 atm, x=0.1.  A step
+is one full spectrum from the device-resident line list: axis reduction,
+axis set-up, LDM fill, forward FFT pass, fused lineshape/reduction FFT pass
+and inverse FFT pass.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0).  Other workloads:
+  --workload co_5k | co2_5m | h2o_100m      single spectra (lines/s)
+  --workload batch_256                      256 requests in one launch set
+                                            (spectra/s), hipGraph replay
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "lines/s (LDM fill+FFT) at ≤1e-4 rel err; batched spectra/s + p99 latency"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="co2_5m")
+    ap.add_argument("--lines", type=int, default=0, help="override the line count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-lines", type=int, default=0, help="lines in the CPU baseline sample (0 = auto)")
+    ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step")
+    return ap.parse_args()
+
+
+def stage_bytes(n_lines, info, single_iso=True):
+    """Algorithmic HBM bytes per launch of each kernel of one request.
+
+    axis    : 12 B/line (g_air, g_self, n_air f32) [+ 12 B/line nu0+iso with several isotopologues]
+    fill    : 28 B/line (nu0 f64, S_ref, El, g_air, g_self, n_air f32) + 4*M*(N_v+1) LDM write
+    fft_a   : 4*M*(N_v+1) LDM read + 8*P*L packed-row spectra write
+    fft_b   : 8*P*L read + 8*L write
+    fft_c   : 8*L read + 4*N_v abscoeff write
+    """
+    M = int(info.n_g * info.n_l)
+    P = (M + 1) // 2
+    L = int(info.fft_len)
+    V = int(info.n_v) + 1
+    return {
+        "axis": n_lines * (12 if single_iso else 24),
+        "finalize": 0,
+        "fill": n_lines * 28 + 4 * M * V,
+        "fft_a": 4 * M * V + 8 * P * L,
+        "fft_b": 8 * P * L + 8 * L,
+        "fft_c": 8 * L + 4 * int(info.n_v),
+    }
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc
+    summary, if present (profiles/<kernel>_<workload>_traffic.txt)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(workload, lines_all, cond, n_sample):
+    """FP64 NumPy restatement of RADIS's CPU LDM-fft path (the oracle) on a
+    bounded sample: the first ``n_sample`` lines of the same list, same grid."""
+    import numpy as np
+
+    from oracle import ldm_oracle as oracle
+
+    n_sample = min(n_sample, len(lines_all))
+    if n_sample == len(lines_all):
+        sub = lines_all
+    else:
+        idx = np.sort(np.random.default_rng(0).choice(len(lines_all), n_sample, replace=False))
+        sub = lines_all.subset(idx)
+    t0 = time.perf_counter()
+    oracle.eq_abscoeff(sub.as_dict(), cond.wmin, cond.wmax, cond.wstep, cond.Tgas, cond.pressure_bar,
+                       cond.mole_fraction, cond.dxG, cond.dxL, Tref=cond.Tref)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n_sample / dt,
+        "unit": "lines/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n_sample} of the {len(lines_all)} {workload} lines, full "
+                  f"{cond.wmin:g}-{cond.wmax:g} cm-1 grid at wstep {cond.wstep:g}, L=2*N_v, one spectrum, "
+                  f"{dt:.1f} s on {platform.processor() or platform.machine()}",
+    }
+
+
+def bench_single(args):
+    import numpy as np
+    import torch
+
+    from radis_amd import engine, workloads
+
+    lines, cond = workloads.workload(args.workload, n=args.lines or None)
+    n_lines = len(lines)
+    dev = engine.Device(0)
+    db = engine.LineDatabase(dev, lines)
+    req = engine.request_from_conditions(db, cond)
+    batch = engine.SpectrumBatch(db, [req])
+    outs = batch.alloc_outputs()
+    stream = torch.cuda.current_stream()
+    if args.graph:
+        batch.capture(outs)
+    step = (lambda: batch.replay(stream)) if args.graph else (lambda: batch.run(outs, stream))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    inf = batch.info(0)
+    if inf.status:
+        raise RuntimeError(f"device status {inf.status}")
+    n_win = int(inf.line_end - inf.line_begin)
+    if not args.graph:
+        batch.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = batch.stage_ms() if not args.graph else None
+    ms = dt / args.steps * 1e3
+    value = n_win * args.steps / dt
+    bytes_ = stage_bytes(n_win, inf, single_iso=not db.has_iso)
+    roof = None
+    if stages:
+        dom = max(stages, key=stages.get)
+        achieved = bytes_[dom] / (stages[dom] * 1e-3) / 1e9
+        roof = {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(dom, args.workload),
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+            "stage_GBps": {k: round(bytes_[k] / (v * 1e-3) / 1e9, 1) for k, v in stages.items() if v > 0},
+            "fill_atomics_per_s": round(8 * n_win / (stages["fill"] * 1e-3), 1),
+        }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "lines/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 indices, Boltzmann, Q ratio)",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "lines": n_lines,
+            "lines_in_window": n_win,
+            "grid_points": int(inf.n_v),
+            "fft_len": int(inf.fft_len),
+            "n1_n2": [int(inf.n1), int(inf.n2)],
+            "N_G": int(inf.n_g),
+            "N_L": int(inf.n_l),
+            "T_K": cond.Tgas,
+            "p_bar": cond.pressure_bar,
+            "x": cond.mole_fraction,
+            "wstep": cond.wstep,
+            "graph": bool(args.graph),
+            "parallelism": "single-gpu",
+        },
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        n_cpu = args.cpu_lines or {"co_5k": 5000, "co2_5m": 5_000_000, "h2o_100m": 2_000_000}.get(args.workload,
+                                                                                                 500_000)
+        out["cpu_baseline"] = cpu_baseline(args.workload, lines, cond, n_cpu)
+    batch.close()
+    db.close()
+    dev.close()
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(every box here has one MI355X)", file=sys.stderr)
+        sys.exit(2)
+    out = bench_single(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

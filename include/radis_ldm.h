@@ -1,0 +1,174 @@
+/*
+ * radis_ldm.h — C ABI of the MI355X-native DIT/LDM spectral-synthesis path.
+ *
+ * Plain C: pointers and sizes only, no C++ or torch types.  Every function
+ * returns 0 on success and a negative RADIS_LDM_E* code on failure; the
+ * message of the last failure on the calling thread is returned by
+ * radis_ldm_last_error().  No exception crosses this boundary.
+ *
+ * Object model (mirrors the reference's gpuApp life cycle,
+ * radis/gpu/gpu.py:35-380):
+ *
+ *   ctx    one HIP device                         (GPUApplication, gpu.py:102)
+ *   db     a device-resident, nu0-sorted line list (database_d upload, gpu.py:190-203)
+ *   batch  one or more spectrum requests over a db with their device
+ *          workspace, job table and optional captured hipGraph
+ *          (command buffer recording, gpu.py:255-284; iterate 302-370)
+ *
+ * Semantics follow RADIS's CPU LDM-fft path (SpectrumFactory.eq_spectrum
+ * with broadening_method='fft', optimization='simple'): p in bar converted
+ * to atm for widths and shifts, exact per-request LDM axes from the lines'
+ * min/max widths, np.interp grid indices (bit-exact), alias folding of the
+ * Voigt Fourier templates.  See the library's design notes for every
+ * departure from the reference GPU shaders.
+ */
+#ifndef RADIS_LDM_H
+#define RADIS_LDM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RADIS_LDM_ABI_VERSION 1
+
+enum {
+    RADIS_LDM_OK = 0,
+    RADIS_LDM_EINVAL = -1,   /* invalid argument (shape, range, unsorted nu0 ...) */
+    RADIS_LDM_EHIP = -2,     /* HIP runtime error */
+    RADIS_LDM_ENOMEM = -3,   /* device allocation failed or exceeds the byte budget */
+    RADIS_LDM_ERANGE = -4,   /* grid or LDM exceeds a compiled limit */
+    RADIS_LDM_EDEVICE = -5,  /* device-side check failed (see radis_ldm_batch_status) */
+};
+
+typedef struct radis_ldm_ctx radis_ldm_ctx;
+typedef struct radis_ldm_db radis_ldm_db;
+typedef struct radis_ldm_batch radis_ldm_batch;
+
+/* Host-side line list, structure of arrays.  nu0 must be sorted ascending.
+ * Optional columns may be NULL.  Replaces the database arrays handed to
+ * gpuApp (radis/gpu/gpu.py:36-54, 190-203): v0, da, na, S0, El, gamma_arr,
+ * iso, Mm_arr.  Intensities are tabulated S(Tref) as used by the CPU path
+ * (radis/lbl/base.py:2664-2674). */
+typedef struct {
+    const double *nu0;        /* [n] line centre, cm-1 (sorted ascending)            */
+    const float *S_ref;       /* [n] line intensity at Tref, cm/molecule               */
+    const float *El;          /* [n] lower-state energy, cm-1                          */
+    const float *g_air;       /* [n] air-broadening HWHM at Tref, cm-1/atm             */
+    const float *g_self;      /* [n] self-broadening HWHM at Tref, cm-1/atm            */
+    const float *n_air;       /* [n] temperature exponent of g_air                    */
+    const float *n_self;      /* [n] or NULL: temperature exponent of g_self (NULL: n_air) */
+    const float *delta;       /* [n] or NULL: air pressure shift, cm-1/atm (NULL: none) */
+    const uint8_t *iso;       /* [n] or NULL: isotopologue slot (NULL: all slot 1)      */
+    int64_t n;                /* number of lines                                        */
+    const double *molar_mass; /* [n_iso_slots] g/mol, indexed by iso (slot 0 unused)    */
+    int32_t n_iso_slots;      /* <= RADIS_LDM_MAX_ISO                                   */
+} radis_ldm_lines;
+
+#define RADIS_LDM_MAX_ISO 16
+
+/* One spectrum request.  Replaces the per-iteration state of gpuApp.iterate
+ * (radis/gpu/gpu.py:302-357, params.py:120-155) and the grid of
+ * SpectrumFactory (radis/lbl/factory.py:2560-2617). */
+typedef struct {
+    double wmin, wmax, wstep; /* grid = arange(wmin, wmax+wstep, wstep); lines with wmin<=nu0<=wmax */
+    double Tgas;              /* K                                                      */
+    double pressure_bar;      /* bar                                                    */
+    double mole_fraction;     /* self mole fraction; air = 1 - x                        */
+    double Tref;              /* K (reference temperature of S_ref and widths)           */
+    double dxG, dxL;          /* LDM log-steps (RADIS Parameters.dxG/dxL)               */
+    double folding_thresh;    /* alias folding threshold (RADIS default 1e-6)           */
+    double q_ratio[RADIS_LDM_MAX_ISO]; /* Q(Tref)/Q(Tgas) per iso slot (host evaluates Q) */
+    int64_t fft_len;          /* 0 = planner choice (smallest supported L >= 2*N_v)     */
+    int64_t n_v;              /* 0 = len(arange(wmin, wmax+wstep, wstep)); else the grid
+                                 is wmin + i*((wmin+wstep)-wmin), i < n_v (gpuApp's Nv) */
+    double line_lo, line_hi;  /* if line_hi > line_lo: lines with line_lo<=nu0<=line_hi,
+                                 otherwise wmin<=nu0<=wmax (lbl/loader.py:2707)          */
+} radis_ldm_request;
+
+/* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
+typedef struct {
+    int64_t n_v;      /* grid points (output length)              */
+    int64_t fft_len;  /* L                                         */
+    int32_t n1, n2;   /* four-step split L = n1*n2                */
+    int64_t line_begin, line_end; /* window line range in the db   */
+    int32_t m_cap;    /* LDM rows allocated (upper bound of N_G*N_L) */
+    int32_t tile_w;   /* fill tile width in grid points            */
+    int32_t scale_log2; /* output = stored value * 2**scale_log2   */
+    int32_t n_g, n_l; /* device-computed LDM axis sizes (valid after a run has completed) */
+    int32_t status;   /* device status of the last completed run (0 = ok) */
+} radis_ldm_request_info;
+
+/* --- context -------------------------------------------------------------- */
+/* Replaces GPUApplication.__init__ device selection (vulkan_compute_lib.py:26-72). */
+int radis_ldm_ctx_create(int device, radis_ldm_ctx **out);
+void radis_ldm_ctx_destroy(radis_ldm_ctx *ctx);
+/* Device-memory budget for workspace admission (bytes; 0 = no limit). */
+int radis_ldm_ctx_set_budget(radis_ldm_ctx *ctx, int64_t bytes);
+int radis_ldm_ctx_bytes_in_use(const radis_ldm_ctx *ctx, int64_t *bytes);
+
+/* --- line database -------------------------------------------------------- */
+/* Replaces the database_d upload of gpuApp.__init__ (gpu.py:174-203). */
+int radis_ldm_db_upload(radis_ldm_ctx *ctx, const radis_ldm_lines *lines, radis_ldm_db **out);
+void radis_ldm_db_destroy(radis_ldm_db *db);
+int radis_ldm_db_size(const radis_ldm_db *db, int64_t *n_lines);
+
+/* --- planning helpers ----------------------------------------------------- */
+/* Grid length of arange(wmin, wmax+wstep, wstep) and the planner's FFT length. */
+int radis_ldm_plan_grid(double wmin, double wmax, double wstep, int64_t *n_v, int64_t *fft_len);
+
+/* --- batches -------------------------------------------------------------- */
+/* Plan n requests over db and allocate their workspace.  Replaces gpuApp's
+ * buffer set-up and command recording (gpu.py:205-297).  A request's
+ * conditions may later be changed with radis_ldm_batch_set_request as long
+ * as the grid (wmin, wmax, wstep, fft_len) is unchanged; the workspace grows
+ * by 1.5x when the new LDM size needs it (vulkan_compute_lib.py:1308-1323). */
+int radis_ldm_batch_create(radis_ldm_ctx *ctx, const radis_ldm_db *db, int n,
+                           const radis_ldm_request *reqs, radis_ldm_batch **out);
+int radis_ldm_batch_set_request(radis_ldm_batch *b, int i, const radis_ldm_request *req);
+/* Enqueue the whole pipeline for every request on `stream` (a hipStream_t,
+ * NULL = the null stream).  out_dev[i] is a device pointer to n_v floats.
+ * Nothing is synchronised; the call allocates nothing in steady state.
+ * Replaces gpuApp.iterate's run() (gpu.py:352-357). */
+int radis_ldm_batch_run(radis_ldm_batch *b, float *const *out_dev, void *stream);
+/* Capture the pipeline into a hipGraph bound to out_dev and replay it.
+ * Conditions changed with _set_request are picked up by later replays. */
+int radis_ldm_batch_graph_capture(radis_ldm_batch *b, float *const *out_dev, void *stream);
+int radis_ldm_batch_graph_replay(radis_ldm_batch *b, void *stream);
+/* Sizes and device results of request i.  n_g/n_l/status are read back
+ * from the device (synchronises `stream`).  Replaces gpuApp.get_griddims
+ * (gpu.py:372-374). */
+int radis_ldm_batch_info(radis_ldm_batch *b, int i, radis_ldm_request_info *info, void *stream);
+int radis_ldm_batch_workspace_bytes(const radis_ldm_batch *b, int64_t *bytes);
+void radis_ldm_batch_destroy(radis_ldm_batch *b);
+
+/* Per-kernel device timings of the last run made with timing enabled,
+ * in milliseconds, in pipeline order (axis, finalize, fill, fft_a,
+ * fft_b, fft_c).  Replaces GPUApplication.get_timestamps
+ * (vulkan_compute_lib.py:973-1024). */
+#define RADIS_LDM_N_STAGES 6
+int radis_ldm_batch_enable_timing(radis_ldm_batch *b, int enable);
+int radis_ldm_batch_stage_ms(radis_ldm_batch *b, float *ms_out);
+
+/* --- diagnostics ---------------------------------------------------------- */
+/* Per-line values the fill kernel scatters, for every line of request i's
+ * window (line_end - line_begin entries, device pointers), computed with the
+ * LDM axes of the last run: grid index k0 and fraction tv, log-wG index l0
+ * and fraction tg, log-wL index m0 and fraction tl, and the intensity
+ * I = S(T) * number density * 2**-scale_log2.  Counterpart of the index and
+ * weight arrays of _apply_lineshape_LDM (radis/lbl/broadening.py:2253-2318). */
+int radis_ldm_batch_line_terms(radis_ldm_batch *b, int i, int64_t *k0, int32_t *l0, int32_t *m0, float *tv,
+                               float *tg, float *tl, float *I, void *stream);
+/* Copy request i's LDM as filled by the last run into dst (device,
+ * row-major [n_g*n_l][n_v+1], rows r = l*N_L + m); the counterpart of the
+ * LDM array of radis/lbl/broadening.py:2340-2346 (transposed). */
+int radis_ldm_batch_copy_ldm(radis_ldm_batch *b, int i, float *dst, void *stream);
+
+const char *radis_ldm_last_error(void);
+int radis_ldm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RADIS_LDM_H */

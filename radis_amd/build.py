@@ -1,0 +1,60 @@
+"""Build recipe of the HIP library (gfx950), run in-tree.
+
+``python -m radis_amd.build`` or ``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "ldm_capi.hip")
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("ldm_capi.hip", "ldm_job.h", "ldm_kernels.h", "ldm_fft.h")] + [
+    os.path.join(ROOT, "include", "radis_ldm.h")
+]
+OUT = os.path.join(HERE, "_lib", "libradis_ldm.so")
+ARCH = os.environ.get("RADIS_AMD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-mcode-object-version=5",
+    # packed FP32 instructions stay off (v_pk_*_f32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build_library(verbose=False):
+    """Compile libradis_ldm.so for gfx950 (always recompiles)."""
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = [l for l in (p.stdout + p.stderr).splitlines() if "packed-fp32-ops" not in l]
+    if p.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
+    if verbose and msgs:
+        print("\n".join(msgs))
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build_library(verbose="-v" in sys.argv))

@@ -1,0 +1,1020 @@
+// Host side of the C ABI declared in include/radis_ldm.h: device context,
+// device-resident line lists, FFT planning, per-request workspace, the job
+// table and the launch sequence of the LDM pipeline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "radis_ldm.h"
+#include "ldm_job.h"
+#include "ldm_kernels.h"
+
+namespace {
+
+// ---- constants: radis/phys/constants.py:22-78 ------------------------------
+const double K_H = 6.62607015e-34;
+const double K_C = 2.99792458e8;
+const double K_KB = 1.3806490e-23;
+const double K_NA = 6.02214076e23;
+const double K_KB_CGS = 1.380648813e-16;
+const double HC_K = K_H * K_C / K_KB * 100;  // constants.py:69
+const double ATM_PER_BAR = 1.01325;          // lbl/broadening.py:1063
+
+const int STAT_BLOCK = 4096;             // lines per host-side statistics block
+const int FILL_LDS_TARGET = 56 * 1024;   // bytes of LDS for one fill tile (plus 2 KB of axis tables)
+const int MIN_TILE_W = 16, MAX_TILE_W = 4096;
+const int N1_MAX = 1024, N2_MAX = 8192;
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(RADIS_LDM_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int ctz(long long x) {
+  int n = 0;
+  while (x > 0 && !(x & 1)) { x >>= 1; ++n; }
+  return n;
+}
+int floor_log2(long long x) {
+  int n = -1;
+  while (x > 0) { x >>= 1; ++n; }
+  return n;
+}
+
+}  // namespace
+
+// ============================================================================
+// objects
+// ============================================================================
+// Contexts and line lists are reference counted by the objects built on
+// them: destroying a parent while children are alive only marks it closed,
+// and the last child to go releases it.
+struct radis_ldm_ctx {
+  int device = 0;
+  int64_t budget = 0;
+  int64_t in_use = 0;
+  int refs = 0;
+  bool closed = false;
+};
+
+struct radis_ldm_db {
+  radis_ldm_ctx* ctx = nullptr;
+  int refs = 0;
+  bool closed = false;
+  int64_t n = 0;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  LdmDb dev{};
+  bool has_iso = false, has_nself = false, has_delta = false;
+  double molar_mass[LDM_MAX_ISO] = {0};
+  int n_iso_slots = 0;
+  unsigned iso_mask = 0;
+  double max_abs_delta = 0.0;
+  std::vector<double> h_nu0;
+  // per-block statistics used for workspace and scale bounds
+  std::vector<float> ga_min, ga_max, gs_min, gs_max, na_min, na_max, ns_min, ns_max, e_min, e_max, s_max;
+};
+
+namespace {
+
+void ctx_release(radis_ldm_ctx* ctx) {
+  if (--ctx->refs == 0 && ctx->closed) delete ctx;
+}
+
+void db_free(radis_ldm_db* db) {
+  if (db->mem) {
+    (void)hipFree(db->mem);
+    db->ctx->in_use -= (int64_t)db->bytes;
+  }
+  radis_ldm_ctx* ctx = db->ctx;
+  delete db;
+  if (ctx) ctx_release(ctx);
+}
+
+void db_release(radis_ldm_db* db) {
+  if (--db->refs == 0 && db->closed) db_free(db);
+}
+
+struct FftPlan {
+  int64_t L = 0;
+  int n1 = 0, n2 = 0;
+  std::vector<int> r1, r2;
+};
+
+struct Req {
+  radis_ldm_request r{};
+  int64_t nv = 0;
+  double delta_w = 0;
+  FftPlan plan;
+  int64_t i0 = 0, i1 = 0;
+  int m_need = 0;      // bound for the current conditions
+  int m_cap = 0;       // allocated rows
+  int tile_w = 0, n_tiles = 0;
+  int64_t stride = 0;
+  int scale_log2 = 0;
+  int bA = 0, bB = 0;
+  size_t off_ldm = 0, off_y = 0, off_b = 0, off_tiles = 0, off_rows = 0;
+  const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
+};
+
+}  // namespace
+
+struct radis_ldm_batch {
+  radis_ldm_ctx* ctx = nullptr;
+  radis_ldm_db* db = nullptr;
+  int n = 0;
+  std::vector<Req> reqs;
+  // two pinned job tables / result slots so that run k only waits for run k-2
+  LdmJob* jobs_h[2] = {nullptr, nullptr};
+  LdmJob* jobs_d = nullptr;
+  LdmAxes* axes_d = nullptr;
+  LdmResult* res_h[2] = {nullptr, nullptr};
+  LdmResult* res_d = nullptr;
+  int next_slot = 0, last_slot = 0;
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  void* twmem = nullptr;
+  size_t tw_bytes = 0;
+  int4* items_d = nullptr;
+  int n_axis = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
+  size_t fill_lds = 0;
+  std::vector<float*> outs;        // outputs of the last eager run
+  std::vector<float*> graph_outs;  // outputs bound to the captured graph
+  hipEvent_t ev_done[2] = {nullptr, nullptr};
+  bool pending[2] = {false, false};
+  bool timing = false;
+  bool slot_timed[2] = {false, false};
+  hipEvent_t st_ev[2][RADIS_LDM_N_STAGES + 1] = {};
+  double stage_sum[RADIS_LDM_N_STAGES] = {0};
+  int stage_count = 0;
+  hipStream_t own_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+};
+
+namespace {
+
+// ---- FFT planning ----------------------------------------------------------
+std::vector<int> smooth16_list(int maxv) {
+  // 16*a with a 7-smooth, <= maxv
+  std::vector<int> out;
+  for (int a = 1; 16 * a <= maxv; ++a) {
+    int x = a;
+    for (int p : {2, 3, 5, 7})
+      while (x % p == 0) x /= p;
+    if (x == 1) out.push_back(16 * a);
+  }
+  return out;
+}
+
+bool radices_of(int n, std::vector<int>& r) {
+  r.clear();
+  for (int p : {16, 8, 4, 2, 7, 5, 3}) {
+    while (n % p == 0) {
+      r.push_back(p);
+      n /= p;
+    }
+  }
+  return n == 1 && (int)r.size() <= LDM_MAX_RAD;
+}
+
+int plan_fft(int64_t nv, int64_t want, FftPlan& out) {
+  static const std::vector<int> c1 = smooth16_list(N1_MAX);
+  static const std::vector<int> c2 = smooth16_list(N2_MAX);
+  const int64_t need = 2 * nv;
+  int64_t bestL = -1;
+  double bestScore = 1e300;
+  int b1 = 0, b2 = 0;
+  for (int n1 : c1) {
+    int64_t n2min = (need + n1 - 1) / n1;
+    if (want > 0) {
+      if (want % n1) continue;
+      int64_t n2 = want / n1;
+      if (!std::binary_search(c2.begin(), c2.end(), (int)std::min<int64_t>(n2, 1 << 30))) continue;
+      double score = std::fabs(std::log2((double)n1) - 9.0);
+      if (score < bestScore) { bestScore = score; bestL = want; b1 = n1; b2 = (int)n2; }
+      continue;
+    }
+    auto itn = std::lower_bound(c2.begin(), c2.end(), (int)std::min<int64_t>(n2min, 1 << 30));
+    if (itn == c2.end()) continue;
+    int64_t L = (int64_t)n1 * *itn;
+    double score = std::fabs(std::log2((double)n1) - 9.0);
+    if (bestL < 0 || L < bestL || (L == bestL && score < bestScore)) {
+      bestL = L; bestScore = score; b1 = n1; b2 = *itn;
+    }
+  }
+  if (bestL < 0) {
+    if (want > 0)
+      return fail(RADIS_LDM_ERANGE, "fft_len %lld is not n1*n2 with 16|n1<=%d, 16|n2<=%d and 7-smooth cofactors",
+                  (long long)want, N1_MAX, N2_MAX);
+    return fail(RADIS_LDM_ERANGE, "grid of %lld points needs an FFT above %lld points", (long long)nv,
+                (long long)N1_MAX * N2_MAX);
+  }
+  if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);
+  out.L = bestL;
+  out.n1 = b1;
+  out.n2 = b2;
+  if (!radices_of(b1, out.r1) || !radices_of(b2, out.r2)) return fail(RADIS_LDM_ERANGE, "radix factorisation failed");
+  return 0;
+}
+
+// numpy.arange(wmin, wmax + wstep, wstep): length ceil((stop-start)/step),
+// values start + i*delta with delta = (start+step) - start.
+int grid_of(double wmin, double wmax, double wstep, int64_t& nv, double& delta) {
+  if (!(wstep > 0) || !(wmax > wmin) || !std::isfinite(wmin) || !std::isfinite(wmax))
+    return fail(RADIS_LDM_EINVAL, "invalid grid wmin=%g wmax=%g wstep=%g", wmin, wmax, wstep);
+  const double stop = wmax + wstep;
+  const double len = std::ceil((stop - wmin) / wstep);
+  if (!(len >= 2) || len > 1e9) return fail(RADIS_LDM_EINVAL, "grid length %g out of range", len);
+  nv = (int64_t)len;
+  delta = (wmin + wstep) - wmin;
+  return 0;
+}
+
+int validate_request(const radis_ldm_request& r) {
+  if (!(r.Tgas > 0) || !(r.Tref > 0)) return fail(RADIS_LDM_EINVAL, "temperatures must be positive");
+  if (!(r.pressure_bar > 0)) return fail(RADIS_LDM_EINVAL, "pressure must be positive");
+  if (!(r.mole_fraction >= 0) || !(r.mole_fraction <= 1)) return fail(RADIS_LDM_EINVAL, "mole_fraction must be in [0, 1]");
+  if (!(r.dxG > 0) || !(r.dxL > 0)) return fail(RADIS_LDM_EINVAL, "dxG and dxL must be positive");
+  if (!(r.folding_thresh > 0)) return fail(RADIS_LDM_EINVAL, "folding_thresh must be positive");
+  return 0;
+}
+
+double dopp_factor(double T, double M) {
+  // sqrt((2 * Na * k_b_CGS * Tgas * ln(2)) / molar_mass), lbl/broadening.py:170-172
+  return std::sqrt((2 * K_NA * K_KB_CGS * T * std::log(2.0)) / M);
+}
+
+// Upper bound of N_G*N_L and the intensity scale for the request's
+// conditions, from per-block statistics of the window's lines.
+int bound_request(const radis_ldm_db& db, Req& R) {
+  const radis_ldm_request& r = R.r;
+  if (R.i1 <= R.i0) {
+    R.m_need = 1;
+    R.scale_log2 = 0;
+    return 0;
+  }
+  const int b0 = (int)(R.i0 / STAT_BLOCK), b1 = (int)((R.i1 - 1) / STAT_BLOCK);
+  double ga0 = 1e300, ga1 = -1e300, gs0 = 1e300, gs1 = -1e300, na0 = 1e300, na1 = -1e300;
+  double ns0 = 1e300, ns1 = -1e300, e0 = 1e300, e1 = -1e300, smax = 0;
+  for (int b = b0; b <= b1; ++b) {
+    ga0 = std::min(ga0, (double)db.ga_min[b]); ga1 = std::max(ga1, (double)db.ga_max[b]);
+    gs0 = std::min(gs0, (double)db.gs_min[b]); gs1 = std::max(gs1, (double)db.gs_max[b]);
+    na0 = std::min(na0, (double)db.na_min[b]); na1 = std::max(na1, (double)db.na_max[b]);
+    ns0 = std::min(ns0, (double)db.ns_min[b]); ns1 = std::max(ns1, (double)db.ns_max[b]);
+    e0 = std::min(e0, (double)db.e_min[b]); e1 = std::max(e1, (double)db.e_max[b]);
+    smax = std::max(smax, (double)db.s_max[b]);
+  }
+  const double p_atm = r.pressure_bar / ATM_PER_BAR;
+  const double lr = std::log(r.Tref / r.Tgas);
+  auto rlo = [&](double a, double b) { return std::min(std::exp(a * lr), std::exp(b * lr)); };
+  auto rhi = [&](double a, double b) { return std::max(std::exp(a * lr), std::exp(b * lr)); };
+  const double x = r.mole_fraction;
+  const double wl_lo = 2 * p_atm * (rlo(na0, na1) * ga0 * (1 - x) + rlo(ns0, ns1) * gs0 * x) * (1 - 1e-9);
+  const double wl_hi = 2 * p_atm * (rhi(na0, na1) * ga1 * (1 - x) + rhi(ns0, ns1) * gs1 * x) * (1 + 1e-9);
+  if (!(wl_lo > 0)) return fail(RADIS_LDM_EINVAL, "zero Lorentzian width in the window (check g_air/g_self and mole fraction)");
+  double dmin = 1e300, dmax = 0;
+  for (int s = 0; s < db.n_iso_slots; ++s) {
+    if (!(db.iso_mask & (1u << s))) continue;
+    const double d = dopp_factor(r.Tgas, db.molar_mass[s]);
+    dmin = std::min(dmin, d);
+    dmax = std::max(dmax, d);
+  }
+  const double wg_lo = 2 * (db.h_nu0[R.i0] / 2.99792458e10) * dmin * (1 - 1e-9);
+  const double wg_hi = 2 * (db.h_nu0[R.i1 - 1] / 2.99792458e10) * dmax * (1 + 1e-9);
+  if (!(wg_lo > 0)) return fail(RADIS_LDM_EINVAL, "non-positive Gaussian width (nu0 <= 0 in window)");
+  const double ngf = std::ceil((std::log(wg_hi + 1e-4) - std::log(wg_lo)) / r.dxG) + 2;
+  const double nlf = std::ceil((std::log(wl_hi + 1e-4) - std::log(wl_lo)) / r.dxL) + 2;
+  if (ngf > LDM_MAX_AX || nlf > LDM_MAX_AX)
+    return fail(RADIS_LDM_ERANGE, "LDM axes too long (N_G<=%g, N_L<=%g; limit %d): increase dxG/dxL", ngf, nlf, LDM_MAX_AX);
+  R.m_need = (int)(ngf * nlf);
+  // intensity scale: log2 of an upper bound of S(T) * density
+  double lb = -1e300;
+  if (smax > 0) {
+    const double cb = -HC_K * (1 / r.Tgas - 1 / r.Tref);
+    double qmax = 0;
+    for (int s = 0; s < db.n_iso_slots; ++s)
+      if (db.iso_mask & (1u << s)) qmax = std::max(qmax, r.q_ratio[s]);
+    const double numin = std::max(db.h_nu0[R.i0], 1e-3);
+    const double stim = (r.Tgas >= r.Tref) ? 1.0 : 1.0 / (1.0 - std::exp(-HC_K * numin / r.Tref));
+    const double dens = r.mole_fraction * ((r.pressure_bar * 1e5) / (K_KB * r.Tgas)) * 1e-6;
+    if (qmax > 0 && dens > 0)
+      lb = std::log2(smax) + std::log2(qmax) + std::max(cb * e0, cb * e1) / std::log(2.0) + std::log2(stim) +
+           std::log2(dens);
+  }
+  R.scale_log2 = (lb > 80 || (lb < -80 && lb > -1e299)) ? (int)std::lround(lb) - 40 : 0;
+  return 0;
+}
+
+int choose_tile(int m_cap) {
+  int w = MAX_TILE_W;
+  while (w > MIN_TILE_W && (size_t)m_cap * w * 4 > (size_t)FILL_LDS_TARGET) w >>= 1;
+  return w;
+}
+
+void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJob& J) {
+  const radis_ldm_request& r = R.r;
+  std::memset(&J, 0, sizeof J);
+  J.wmin = r.wmin;
+  J.delta_w = R.delta_w;
+  J.wstep = r.wstep;
+  J.nv = R.nv;
+  J.v = R.nv + 1;
+  J.L = R.plan.L;
+  J.i0 = R.i0;
+  J.i1 = R.i1;
+  J.p_atm = r.pressure_bar / ATM_PER_BAR;
+  J.x_self = r.mole_fraction;
+  J.x_air = 1 - r.mole_fraction;
+  J.log_r = std::log(r.Tref / r.Tgas);
+  J.c_boltz = -HC_K * (1 / r.Tgas - 1 / r.Tref);
+  const double dens = r.mole_fraction * ((r.pressure_bar * 1e5) / (K_KB * r.Tgas)) * 1e-6;  // factory.py:908
+  J.i_scale = std::ldexp(dens, -R.scale_log2);
+  for (int s = 0; s < LDM_MAX_ISO; ++s) {
+    J.dopp[s] = (s < db.n_iso_slots && db.molar_mass[s] > 0) ? dopp_factor(r.Tgas, db.molar_mass[s]) : 0.0;
+    J.qratio[s] = r.q_ratio[s];
+  }
+  J.dxG = r.dxG;
+  J.dxL = r.dxL;
+  J.fold_thresh = r.folding_thresh;
+  J.shift_margin = (db.has_delta ? db.max_abs_delta * J.p_atm : 0.0) + 2 * R.delta_w;
+  J.c_stim_T = (float)(-HC_K / r.Tgas);
+  J.c_stim_R = (float)(-HC_K / r.Tref);
+  J.out_scale = (float)(1.0 / ((double)R.plan.L * r.wstep));
+  J.inv_Lws = (float)(1.0 / ((double)R.plan.L * r.wstep));
+  J.half_inv_ws = (float)(0.5 / r.wstep);
+  J.single_iso = db.has_iso ? 0 : 1;
+  J.n1 = R.plan.n1;
+  J.n2 = R.plan.n2;
+  J.nrad1 = (int)R.plan.r1.size();
+  J.nrad2 = (int)R.plan.r2.size();
+  for (int i = 0; i < J.nrad1; ++i) J.rad1[i] = R.plan.r1[i];
+  for (int i = 0; i < J.nrad2; ++i) J.rad2[i] = R.plan.r2[i];
+  J.bA = R.bA;
+  J.bB = R.bB;
+  J.m_cap = R.m_cap;
+  J.tile_w = R.tile_w;
+  J.n_tiles = R.n_tiles;
+  J.ldm_stride = R.stride;
+  J.ldm = (float*)(work + R.off_ldm);
+  J.ybuf = (float2*)(work + R.off_y);
+  J.bbuf = (float2*)(work + R.off_b);
+  J.out = out;
+  J.tiles = (long long*)(work + R.off_tiles);
+  J.nfold = (int*)(work + R.off_rows);
+  J.row_a = (float*)(work + R.off_rows + (size_t)4 * R.m_cap);
+  J.row_b = (float*)(work + R.off_rows + (size_t)8 * R.m_cap);
+  J.row_inv_f0 = (float*)(work + R.off_rows + (size_t)12 * R.m_cap);
+  J.tw1 = R.tw1;
+  J.tw2 = R.tw2;
+  J.twl_hi = R.twl_hi;
+  J.twl_lo = R.twl_lo;
+  J.wl_min_bits = ~0ull;
+  J.wl_max_bits = 0ull;
+  J.wg_min_bits = ~0ull;
+  J.wg_max_bits = 0ull;
+  J.status = 0;
+}
+
+void free_layout(radis_ldm_batch* b) {
+  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->graph_valid = false;
+  if (b->work) { (void)hipFree(b->work); b->ctx->in_use -= (int64_t)b->work_bytes; b->work = nullptr; b->work_bytes = 0; }
+  if (b->twmem) { (void)hipFree(b->twmem); b->ctx->in_use -= (int64_t)b->tw_bytes; b->twmem = nullptr; b->tw_bytes = 0; }
+  if (b->items_d) { (void)hipFree(b->items_d); b->items_d = nullptr; }
+}
+
+// Allocate workspace, twiddle tables and work-item lists for every request.
+int build_layout(radis_ldm_batch* b) {
+  free_layout(b);
+  // --- workspace
+  size_t total = 0;
+  size_t fill_lds = 0;
+  for (Req& R : b->reqs) {
+    R.tile_w = choose_tile(R.m_cap);
+    if ((size_t)R.m_cap * R.tile_w * 4 > (size_t)FILL_LDS_TARGET)
+      return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit the fill tile (max %d rows)", R.m_cap,
+                  FILL_LDS_TARGET / (4 * MIN_TILE_W));
+    const int64_t V = R.nv + 1;
+    R.n_tiles = (int)((V + R.tile_w - 1) / R.tile_w);
+    R.stride = (int64_t)align_up((size_t)V, 64);
+    fill_lds = std::max(fill_lds, (size_t)R.m_cap * R.tile_w * 4);
+    const int n1 = R.plan.n1, n2 = R.plan.n2;
+    R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
+    R.bB = std::min(floor_log2(LDM_CAP / n2), ctz(n1));
+    const int pcap = (R.m_cap + 1) / 2;
+    R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
+    R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
+    R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
+    R.off_tiles = total; total = align_up(total + (size_t)R.n_tiles * 16, 256);
+    R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
+  }
+  // --- twiddle tables, deduplicated by size
+  std::map<int, size_t> sub_off;
+  std::map<int64_t, size_t> l_off;
+  size_t twsz = 0;
+  for (Req& R : b->reqs) {
+    for (int N : {R.plan.n1, R.plan.n2})
+      if (!sub_off.count(N)) { sub_off[N] = twsz; twsz = align_up(twsz + (size_t)N * 8, 256); }
+    if (!l_off.count(R.plan.L)) {
+      l_off[R.plan.L] = twsz;
+      const size_t nhi = (size_t)((R.plan.L + 1023) >> LDM_TW_LO_BITS);
+      twsz = align_up(twsz + (nhi + 1024) * 8, 256);
+    }
+  }
+  if (b->ctx->budget > 0 && b->ctx->in_use + (int64_t)(total + twsz) > b->ctx->budget)
+    return fail(RADIS_LDM_ENOMEM, "workspace of %zu bytes exceeds the device budget (%lld in use of %lld)",
+                total + twsz, (long long)b->ctx->in_use, (long long)b->ctx->budget);
+  if (hipMalloc(&b->work, std::max<size_t>(total, 256)) != hipSuccess) {
+    b->work = nullptr;
+    return fail(RADIS_LDM_ENOMEM, "hipMalloc of %zu workspace bytes failed", total);
+  }
+  b->work_bytes = std::max<size_t>(total, 256);
+  b->ctx->in_use += (int64_t)b->work_bytes;
+  if (hipMalloc(&b->twmem, twsz) != hipSuccess) {
+    b->twmem = nullptr;
+    return fail(RADIS_LDM_ENOMEM, "hipMalloc of %zu twiddle bytes failed", twsz);
+  }
+  b->tw_bytes = twsz;
+  b->ctx->in_use += (int64_t)twsz;
+  {
+    std::vector<float2> h(twsz / 8);
+    const double TWO_PI = 6.283185307179586476925286766559;
+    for (auto& kv : sub_off) {
+      float2* t = h.data() + kv.second / 8;
+      for (int e = 0; e < kv.first; ++e) {
+        const double a = -TWO_PI * (double)e / (double)kv.first;
+        t[e] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+    }
+    for (auto& kv : l_off) {
+      const int64_t L = kv.first;
+      const size_t nhi = (size_t)((L + 1023) >> LDM_TW_LO_BITS);
+      float2* hi = h.data() + kv.second / 8;
+      float2* lo = hi + nhi;
+      for (size_t q = 0; q < nhi; ++q) {
+        const double a = -TWO_PI * (double)((int64_t)(q << LDM_TW_LO_BITS) % L) / (double)L;
+        hi[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+      for (int e = 0; e < 1024; ++e) {
+        const double a = -TWO_PI * (double)e / (double)L;
+        lo[e] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+    }
+    HIPCHK(hipMemcpy(b->twmem, h.data(), twsz, hipMemcpyHostToDevice));
+  }
+  for (Req& R : b->reqs) {
+    char* base = (char*)b->twmem;
+    R.tw1 = (const float2*)(base + sub_off[R.plan.n1]);
+    R.tw2 = (const float2*)(base + sub_off[R.plan.n2]);
+    R.twl_hi = (const float2*)(base + l_off[R.plan.L]);
+    R.twl_lo = R.twl_hi + (size_t)((R.plan.L + 1023) >> LDM_TW_LO_BITS);
+  }
+  // --- work items
+  std::vector<int4> ax, fi, fa, fb, fc;
+  for (int j = 0; j < b->n; ++j) {
+    const Req& R = b->reqs[j];
+    const int64_t nl = R.i1 - R.i0;
+    for (int64_t c = 0; c * LDM_AX_CHUNK < nl; ++c) ax.push_back(make_int4(j, (int)c, 0, 0));
+    for (int t = 0; t < R.n_tiles; ++t) fi.push_back(make_int4(j, t, 0, 0));
+    const int pcap = (R.m_cap + 1) / 2;
+    const int nA = R.plan.n2 >> R.bA;
+    for (int p = 0; p < pcap; ++p)
+      for (int cb = 0; cb < nA; ++cb) fa.push_back(make_int4(j, p, cb, 0));
+    for (int kb = 0; kb < (R.plan.n1 >> R.bB); ++kb) fb.push_back(make_int4(j, kb, 0, 0));
+    for (int cb = 0; cb < nA; ++cb) fc.push_back(make_int4(j, 0, cb, 0));
+  }
+  b->n_axis = (int)ax.size();
+  b->n_fill = (int)fi.size();
+  b->n_a = (int)fa.size();
+  b->n_b = (int)fb.size();
+  b->n_c = (int)fc.size();
+  std::vector<int4> all;
+  for (auto* v : {&ax, &fi, &fa, &fb, &fc}) all.insert(all.end(), v->begin(), v->end());
+  if (all.empty()) all.push_back(make_int4(0, 0, 0, 0));
+  HIPCHK(hipMalloc(&b->items_d, all.size() * sizeof(int4)));
+  HIPCHK(hipMemcpy(b->items_d, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+  b->fill_lds = std::max<size_t>(fill_lds, 4);
+  return 0;
+}
+
+int plan_one(const radis_ldm_db& db, const radis_ldm_request& rq, Req& R) {
+  int rc = validate_request(rq);
+  if (rc) return rc;
+  R.r = rq;
+  rc = grid_of(rq.wmin, rq.wmax, rq.wstep, R.nv, R.delta_w);
+  if (rc) return rc;
+  if (rq.n_v > 0) {
+    if (rq.n_v < 2 || rq.n_v > 1000000000) return fail(RADIS_LDM_EINVAL, "n_v %lld out of range", (long long)rq.n_v);
+    R.nv = rq.n_v;
+  }
+  rc = plan_fft(R.nv, rq.fft_len, R.plan);
+  if (rc) return rc;
+  const bool lw = rq.line_hi > rq.line_lo;
+  const double lo = lw ? rq.line_lo : rq.wmin, hi = lw ? rq.line_hi : rq.wmax;
+  R.i0 = std::lower_bound(db.h_nu0.begin(), db.h_nu0.end(), lo) - db.h_nu0.begin();
+  R.i1 = std::upper_bound(db.h_nu0.begin(), db.h_nu0.end(), hi) - db.h_nu0.begin();
+  if (R.i1 > R.i0 && R.i1 - R.i0 > (int64_t)2000000000)
+    return fail(RADIS_LDM_ERANGE, "window holds more than 2e9 lines");
+  return bound_request(db, R);
+}
+
+// Host-to-device upload of the job table (outside any graph: a captured
+// copy from pinned memory is not guaranteed to re-read the host buffer).
+int upload_jobs(radis_ldm_batch* b, hipStream_t s, int slot) {
+  HIPCHK(hipMemcpyAsync(b->jobs_d, b->jobs_h[slot], sizeof(LdmJob) * b->n, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+int download_results(radis_ldm_batch* b, hipStream_t s, int slot) {
+  HIPCHK(hipMemcpyAsync(b->res_h[slot], b->res_d, sizeof(LdmResult) * b->n, hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+// The kernel sequence of one run (what a graph captures).
+int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
+  const radis_ldm_db& db = *b->db;
+  const int4* items = b->items_d;
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][0], s));
+  if (b->n_axis) hipLaunchKernelGGL(k_axis, dim3(b->n_axis), dim3(256), 0, s, b->jobs_d, items, db.dev);
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][1], s));
+  hipLaunchKernelGGL(k_finalize, dim3(b->n), dim3(64), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][2], s));
+  items += b->n_axis;
+  if (b->n_fill) {
+    hipLaunchKernelGGL(k_tile_bounds, dim3((b->n_fill + 255) / 256), dim3(256), 0, s, b->jobs_d, items, b->n_fill,
+                       db.dev);
+    hipLaunchKernelGGL(k_fill, dim3(b->n_fill), dim3(LDM_FILL_T), b->fill_lds, s, (const LdmJob*)b->jobs_d,
+                       (const LdmAxes*)b->axes_d, items, db.dev);
+  }
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][3], s));
+  items += b->n_fill;
+  if (b->n_a)
+    hipLaunchKernelGGL(k_fft_a, dim3(b->n_a), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d,
+                       (const LdmAxes*)b->axes_d, items);
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][4], s));
+  items += b->n_a;
+  if (b->n_b)
+    hipLaunchKernelGGL(k_fft_b, dim3(b->n_b), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d,
+                       (const LdmAxes*)b->axes_d, items);
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][5], s));
+  items += b->n_b;
+  if (b->n_c) hipLaunchKernelGGL(k_fft_c, dim3(b->n_c), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d, items);
+  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][6], s));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int enqueue(radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
+  int rc = upload_jobs(b, s, slot);
+  if (!rc) rc = launch_kernels(b, s, timed, slot);
+  if (!rc) rc = download_results(b, s, slot);
+  return rc;
+}
+
+// Wait for the run that last used `slot` and fold its stage times into the
+// accumulators.
+int wait_slot(radis_ldm_batch* b, int slot) {
+  if (!b->pending[slot]) return 0;
+  HIPCHK(hipEventSynchronize(b->ev_done[slot]));
+  b->pending[slot] = false;
+  if (b->slot_timed[slot]) {
+    for (int k = 0; k < RADIS_LDM_N_STAGES; ++k) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, b->st_ev[slot][k], b->st_ev[slot][k + 1]));
+      b->stage_sum[k] += ms;
+    }
+    b->stage_count++;
+    b->slot_timed[slot] = false;
+  }
+  return 0;
+}
+
+int wait_pending(radis_ldm_batch* b) {
+  int rc = wait_slot(b, 0);
+  if (!rc) rc = wait_slot(b, 1);
+  return rc;
+}
+
+void refresh_jobs(radis_ldm_batch* b, int slot, const std::vector<float*>& outs) {
+  for (int j = 0; j < b->n; ++j) fill_job(*b->db, b->reqs[j], (char*)b->work, outs[j], b->jobs_h[slot][j]);
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* radis_ldm_last_error(void) { return g_err.c_str(); }
+int radis_ldm_abi_version(void) { return RADIS_LDM_ABI_VERSION; }
+
+int radis_ldm_ctx_create(int device, radis_ldm_ctx** out) {
+  if (!out) return fail(RADIS_LDM_EINVAL, "null out");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(RADIS_LDM_EINVAL, "device %d not in [0, %d)", device, n);
+  HIPCHK(hipSetDevice(device));
+  auto* c = new radis_ldm_ctx();
+  c->device = device;
+  *out = c;
+  return 0;
+}
+
+void radis_ldm_ctx_destroy(radis_ldm_ctx* ctx) {
+  if (!ctx) return;
+  ctx->closed = true;
+  if (ctx->refs == 0) delete ctx;
+}
+
+int radis_ldm_ctx_set_budget(radis_ldm_ctx* ctx, int64_t bytes) {
+  if (!ctx || bytes < 0) return fail(RADIS_LDM_EINVAL, "bad budget");
+  ctx->budget = bytes;
+  return 0;
+}
+
+int radis_ldm_ctx_bytes_in_use(const radis_ldm_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return fail(RADIS_LDM_EINVAL, "null argument");
+  *bytes = ctx->in_use;
+  return 0;
+}
+
+int radis_ldm_plan_grid(double wmin, double wmax, double wstep, int64_t* n_v, int64_t* fft_len) {
+  int64_t nv;
+  double d;
+  int rc = grid_of(wmin, wmax, wstep, nv, d);
+  if (rc) return rc;
+  FftPlan p;
+  rc = plan_fft(nv, 0, p);
+  if (rc) return rc;
+  if (n_v) *n_v = nv;
+  if (fft_len) *fft_len = p.L;
+  return 0;
+}
+
+int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_db** out) {
+  if (!ctx || !L || !out) return fail(RADIS_LDM_EINVAL, "null argument");
+  if (L->n <= 0) return fail(RADIS_LDM_EINVAL, "empty line list");
+  if (!L->nu0 || !L->S_ref || !L->El || !L->g_air || !L->g_self || !L->n_air || !L->molar_mass)
+    return fail(RADIS_LDM_EINVAL, "missing required column");
+  if (L->n_iso_slots < 2 || L->n_iso_slots > RADIS_LDM_MAX_ISO)
+    return fail(RADIS_LDM_EINVAL, "n_iso_slots must be in [2, %d]", RADIS_LDM_MAX_ISO);
+  const int64_t n = L->n;
+  for (int64_t i = 1; i < n; ++i)
+    if (!(L->nu0[i - 1] <= L->nu0[i])) return fail(RADIS_LDM_EINVAL, "nu0 not sorted ascending at line %lld", (long long)i);
+  if (!std::isfinite(L->nu0[0]) || !std::isfinite(L->nu0[n - 1])) return fail(RADIS_LDM_EINVAL, "non-finite nu0");
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* db = new radis_ldm_db();
+  db->ctx = nullptr;  // attached (and counted) once the device copy exists
+  db->n = n;
+  db->has_iso = L->iso != nullptr;
+  db->has_nself = L->n_self != nullptr;
+  db->has_delta = L->delta != nullptr;
+  db->n_iso_slots = L->n_iso_slots;
+  for (int s = 0; s < L->n_iso_slots; ++s) db->molar_mass[s] = L->molar_mass[s];
+  if (db->has_iso) {
+    for (int64_t i = 0; i < n; ++i) {
+      const int s = L->iso[i];
+      if (s <= 0 || s >= L->n_iso_slots || !(L->molar_mass[s] > 0)) {
+        delete db;
+        return fail(RADIS_LDM_EINVAL, "line %lld: iso slot %d has no molar mass", (long long)i, s);
+      }
+      db->iso_mask |= 1u << s;
+    }
+  } else {
+    if (!(L->molar_mass[1] > 0)) { delete db; return fail(RADIS_LDM_EINVAL, "molar_mass[1] must be positive"); }
+    db->iso_mask = 2u;
+  }
+  if (db->has_delta)
+    for (int64_t i = 0; i < n; ++i) db->max_abs_delta = std::max(db->max_abs_delta, (double)std::fabs(L->delta[i]));
+  db->h_nu0.assign(L->nu0, L->nu0 + n);
+  const int nb = (int)((n + STAT_BLOCK - 1) / STAT_BLOCK);
+  for (auto* v : {&db->ga_min, &db->gs_min, &db->na_min, &db->ns_min, &db->e_min}) v->assign(nb, 3.4e38f);
+  for (auto* v : {&db->ga_max, &db->gs_max, &db->na_max, &db->ns_max, &db->e_max}) v->assign(nb, -3.4e38f);
+  db->s_max.assign(nb, 0.0f);
+  for (int64_t i = 0; i < n; ++i) {
+    const int b = (int)(i / STAT_BLOCK);
+    const float ns = L->n_self ? L->n_self[i] : L->n_air[i];
+    db->ga_min[b] = std::min(db->ga_min[b], L->g_air[i]); db->ga_max[b] = std::max(db->ga_max[b], L->g_air[i]);
+    db->gs_min[b] = std::min(db->gs_min[b], L->g_self[i]); db->gs_max[b] = std::max(db->gs_max[b], L->g_self[i]);
+    db->na_min[b] = std::min(db->na_min[b], L->n_air[i]); db->na_max[b] = std::max(db->na_max[b], L->n_air[i]);
+    db->ns_min[b] = std::min(db->ns_min[b], ns); db->ns_max[b] = std::max(db->ns_max[b], ns);
+    db->e_min[b] = std::min(db->e_min[b], L->El[i]); db->e_max[b] = std::max(db->e_max[b], L->El[i]);
+    db->s_max[b] = std::max(db->s_max[b], L->S_ref[i]);
+  }
+  // device arrays: one allocation
+  size_t off[9], tot = 0;
+  const size_t sz[9] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
+                        db->has_nself ? (size_t)n * 4 : 0, db->has_delta ? (size_t)n * 4 : 0,
+                        db->has_iso ? (size_t)n : 0};
+  for (int k = 0; k < 9; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
+  if (ctx->budget > 0 && ctx->in_use + (int64_t)tot > ctx->budget) {
+    delete db;
+    return fail(RADIS_LDM_ENOMEM, "line list of %zu bytes exceeds the device budget", tot);
+  }
+  if (hipMalloc(&db->mem, tot) != hipSuccess) {
+    delete db;
+    return fail(RADIS_LDM_ENOMEM, "hipMalloc of %zu bytes for the line list failed", tot);
+  }
+  db->bytes = tot;
+  db->ctx = ctx;
+  ctx->refs++;
+  ctx->in_use += (int64_t)tot;
+  char* m = (char*)db->mem;
+  const void* src[9] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso};
+  for (int k = 0; k < 9; ++k)
+    if (sz[k]) {
+      hipError_t e = hipMemcpy(m + off[k], src[k], sz[k], hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        radis_ldm_db_destroy(db);
+        return fail(RADIS_LDM_EHIP, "line list upload: %s", hipGetErrorString(e));
+      }
+    }
+  db->dev.nu0 = (const double*)(m + off[0]);
+  db->dev.S_ref = (const float*)(m + off[1]);
+  db->dev.El = (const float*)(m + off[2]);
+  db->dev.g_air = (const float*)(m + off[3]);
+  db->dev.g_self = (const float*)(m + off[4]);
+  db->dev.n_air = (const float*)(m + off[5]);
+  db->dev.n_self = db->has_nself ? (const float*)(m + off[6]) : nullptr;
+  db->dev.delta = db->has_delta ? (const float*)(m + off[7]) : nullptr;
+  db->dev.iso = db->has_iso ? (const uint8_t*)(m + off[8]) : nullptr;
+  *out = db;
+  return 0;
+}
+
+void radis_ldm_db_destroy(radis_ldm_db* db) {
+  if (!db) return;
+  db->closed = true;
+  if (db->refs == 0) db_free(db);
+}
+
+int radis_ldm_db_size(const radis_ldm_db* db, int64_t* n_lines) {
+  if (!db || !n_lines) return fail(RADIS_LDM_EINVAL, "null argument");
+  *n_lines = db->n;
+  return 0;
+}
+
+int radis_ldm_batch_create(radis_ldm_ctx* ctx, const radis_ldm_db* db, int n, const radis_ldm_request* reqs,
+                           radis_ldm_batch** out) {
+  if (!ctx || !db || !reqs || !out || n <= 0) return fail(RADIS_LDM_EINVAL, "bad batch arguments");
+  if (db->ctx != ctx) return fail(RADIS_LDM_EINVAL, "line list belongs to another context");
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* b = new radis_ldm_batch();
+  b->ctx = ctx;
+  b->db = const_cast<radis_ldm_db*>(db);
+  ctx->refs++;
+  b->db->refs++;
+  b->n = n;
+  b->reqs.resize(n);
+  b->outs.assign(n, nullptr);
+  int rc = 0;
+  for (int j = 0; j < n && !rc; ++j) {
+    rc = plan_one(*db, reqs[j], b->reqs[j]);
+    b->reqs[j].m_cap = b->reqs[j].m_need;
+  }
+  if (!rc) rc = build_layout(b);
+  if (!rc) {
+    bool ok = hipMalloc(&b->jobs_d, sizeof(LdmJob) * n) == hipSuccess &&
+              hipMalloc(&b->axes_d, sizeof(LdmAxes) * n) == hipSuccess &&
+              hipMalloc(&b->res_d, sizeof(LdmResult) * n) == hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k) {
+      ok = hipHostMalloc((void**)&b->jobs_h[k], sizeof(LdmJob) * n) == hipSuccess &&
+           hipHostMalloc((void**)&b->res_h[k], sizeof(LdmResult) * n) == hipSuccess &&
+           hipEventCreateWithFlags(&b->ev_done[k], hipEventDisableTiming) == hipSuccess;
+      if (ok) std::memset(b->res_h[k], 0, sizeof(LdmResult) * n);
+      for (auto& e : b->st_ev[k])
+        if (ok && hipEventCreate(&e) != hipSuccess) ok = false;
+    }
+    if (!ok) rc = fail(RADIS_LDM_ENOMEM, "batch bookkeeping allocation failed");
+  }
+  if (rc) {
+    std::string msg = g_err;
+    radis_ldm_batch_destroy(b);
+    g_err = msg;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int radis_ldm_batch_set_request(radis_ldm_batch* b, int i, const radis_ldm_request* req) {
+  if (!b || !req || i < 0 || i >= b->n) return fail(RADIS_LDM_EINVAL, "bad arguments");
+  Req& R = b->reqs[i];
+  if (req->wmin != R.r.wmin || req->wmax != R.r.wmax || req->wstep != R.r.wstep ||
+      (req->fft_len != 0 && req->fft_len != R.plan.L) || req->n_v != R.r.n_v ||
+      req->line_lo != R.r.line_lo || req->line_hi != R.r.line_hi)
+    return fail(RADIS_LDM_EINVAL, "set_request cannot change the grid; create a new batch");
+  int rc = validate_request(*req);
+  if (rc) return rc;
+  rc = wait_pending(b);
+  if (rc) return rc;
+  Req T = R;
+  T.r = *req;
+  T.r.fft_len = R.plan.L;
+  rc = bound_request(*b->db, T);
+  if (rc) return rc;
+  const bool grow = T.m_need > R.m_cap;
+  R = T;
+  if (grow) {
+    R.m_cap = std::max(T.m_need, (int)std::ceil(1.5 * R.m_cap));  // vulkan_compute_lib.py:1308-1313
+    rc = build_layout(b);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int radis_ldm_batch_run(radis_ldm_batch* b, float* const* out_dev, void* stream) {
+  if (!b || !out_dev) return fail(RADIS_LDM_EINVAL, "null argument");
+  for (int j = 0; j < b->n; ++j)
+    if (!out_dev[j]) return fail(RADIS_LDM_EINVAL, "null output pointer for request %d", j);
+  HIPCHK(hipSetDevice(b->ctx->device));
+  const int slot = b->next_slot;
+  int rc = wait_slot(b, slot);
+  if (rc) return rc;
+  for (int j = 0; j < b->n; ++j) b->outs[j] = out_dev[j];
+  refresh_jobs(b, slot, b->outs);
+  hipStream_t s = (hipStream_t)stream;
+  rc = enqueue(b, s, b->timing, slot);
+  if (rc) return rc;
+  b->slot_timed[slot] = b->timing;
+  HIPCHK(hipEventRecord(b->ev_done[slot], s));
+  b->pending[slot] = true;
+  b->last_slot = slot;
+  b->next_slot = slot ^ 1;
+  return 0;
+}
+
+int radis_ldm_batch_graph_capture(radis_ldm_batch* b, float* const* out_dev, void* stream) {
+  if (!b || !out_dev) return fail(RADIS_LDM_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  int rc = wait_pending(b);
+  if (rc) return rc;
+  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->graph_outs.assign(out_dev, out_dev + b->n);
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) {
+    if (!b->own_stream) HIPCHK(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
+    s = b->own_stream;
+  }
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  rc = launch_kernels(b, s, false, 0);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(RADIS_LDM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  b->graph_valid = true;
+  return 0;
+}
+
+int radis_ldm_batch_graph_replay(radis_ldm_batch* b, void* stream) {
+  if (!b) return fail(RADIS_LDM_EINVAL, "null batch");
+  if (!b->graph_valid || !b->gexec) return fail(RADIS_LDM_EINVAL, "no valid captured graph (capture again after a re-plan)");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  const int slot = b->next_slot;
+  int rc = wait_slot(b, slot);
+  if (rc) return rc;
+  refresh_jobs(b, slot, b->graph_outs);
+  hipStream_t s = (hipStream_t)stream;
+  rc = upload_jobs(b, s, slot);
+  if (rc) return rc;
+  HIPCHK(hipGraphLaunch(b->gexec, s));
+  rc = download_results(b, s, slot);
+  if (rc) return rc;
+  b->slot_timed[slot] = false;
+  HIPCHK(hipEventRecord(b->ev_done[slot], s));
+  b->pending[slot] = true;
+  b->last_slot = slot;
+  b->next_slot = slot ^ 1;
+  return 0;
+}
+
+int radis_ldm_batch_info(radis_ldm_batch* b, int i, radis_ldm_request_info* info, void* stream) {
+  (void)stream;
+  if (!b || !info || i < 0 || i >= b->n) return fail(RADIS_LDM_EINVAL, "bad arguments");
+  int rc = wait_pending(b);
+  if (rc) return rc;
+  const Req& R = b->reqs[i];
+  info->n_v = R.nv;
+  info->fft_len = R.plan.L;
+  info->n1 = R.plan.n1;
+  info->n2 = R.plan.n2;
+  info->line_begin = R.i0;
+  info->line_end = R.i1;
+  info->m_cap = R.m_cap;
+  info->tile_w = R.tile_w;
+  info->scale_log2 = R.scale_log2;
+  const LdmResult& res = b->res_h[b->last_slot][i];
+  info->n_g = res.ng;
+  info->n_l = res.nl;
+  info->status = res.status;
+  return 0;
+}
+
+int radis_ldm_batch_workspace_bytes(const radis_ldm_batch* b, int64_t* bytes) {
+  if (!b || !bytes) return fail(RADIS_LDM_EINVAL, "null argument");
+  *bytes = (int64_t)(b->work_bytes + b->tw_bytes);
+  return 0;
+}
+
+int radis_ldm_batch_enable_timing(radis_ldm_batch* b, int enable) {
+  if (!b) return fail(RADIS_LDM_EINVAL, "null batch");
+  b->timing = enable != 0;
+  return 0;
+}
+
+int radis_ldm_batch_stage_ms(radis_ldm_batch* b, float* ms_out) {
+  if (!b || !ms_out) return fail(RADIS_LDM_EINVAL, "null argument");
+  int rc = wait_pending(b);
+  if (rc) return rc;
+  if (b->stage_count == 0) return fail(RADIS_LDM_EINVAL, "no timed run available (enable timing and run)");
+  for (int k = 0; k < RADIS_LDM_N_STAGES; ++k) {
+    ms_out[k] = (float)(b->stage_sum[k] / b->stage_count);
+    b->stage_sum[k] = 0;
+  }
+  b->stage_count = 0;
+  return 0;
+}
+
+int radis_ldm_batch_line_terms(radis_ldm_batch* b, int i, int64_t* k0, int32_t* l0, int32_t* m0, float* tv,
+                               float* tg, float* tl, float* I, void* stream) {
+  if (!b || i < 0 || i >= b->n || !k0 || !l0 || !m0 || !tv || !tg || !tl || !I)
+    return fail(RADIS_LDM_EINVAL, "bad arguments");
+  if (!b->outs[i]) return fail(RADIS_LDM_EINVAL, "run the batch first");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  const Req& R = b->reqs[i];
+  const int64_t n = R.i1 - R.i0;
+  if (n <= 0) return 0;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_line_terms, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const LdmJob*)b->jobs_d,
+                     (const LdmAxes*)b->axes_d, i, b->db->dev, (long long*)k0, (int*)l0, (int*)m0, tv, tg, tl, I);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int radis_ldm_batch_copy_ldm(radis_ldm_batch* b, int i, float* dst, void* stream) {
+  if (!b || i < 0 || i >= b->n || !dst) return fail(RADIS_LDM_EINVAL, "bad arguments");
+  int rc = wait_pending(b);
+  if (rc) return rc;
+  const Req& R = b->reqs[i];
+  const int rows = b->res_h[b->last_slot][i].ng * b->res_h[b->last_slot][i].nl;
+  if (rows == 0) return 0;
+  const size_t cols = (size_t)(R.nv + 1);
+  HIPCHK(hipMemcpy2DAsync(dst, cols * 4, (char*)b->work + R.off_ldm, (size_t)R.stride * 4, cols * 4, rows,
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+void radis_ldm_batch_destroy(radis_ldm_batch* b) {
+  if (!b) return;
+  for (int k = 0; k < 2; ++k)
+    if (b->pending[k] && b->ev_done[k]) (void)hipEventSynchronize(b->ev_done[k]);
+  free_layout(b);
+  for (int k = 0; k < 2; ++k) {
+    if (b->jobs_h[k]) (void)hipHostFree(b->jobs_h[k]);
+    if (b->res_h[k]) (void)hipHostFree(b->res_h[k]);
+    if (b->ev_done[k]) (void)hipEventDestroy(b->ev_done[k]);
+    for (auto& e : b->st_ev[k])
+      if (e) (void)hipEventDestroy(e);
+  }
+  if (b->jobs_d) (void)hipFree(b->jobs_d);
+  if (b->axes_d) (void)hipFree(b->axes_d);
+  if (b->res_d) (void)hipFree(b->res_d);
+  if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+  radis_ldm_ctx* ctx = b->ctx;
+  radis_ldm_db* db = b->db;
+  delete b;
+  if (db) db_release(db);
+  if (ctx) ctx_release(ctx);
+}
+
+}  // extern "C"

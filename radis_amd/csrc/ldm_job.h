@@ -1,0 +1,95 @@
+// Device-side job descriptor shared by the host planner and the kernels.
+//
+// One LdmJob describes one spectrum request: its grid, its window of lines
+// in the device line list, its physical conditions and its workspace.  The
+// host fills it before every run (one H2D copy of the whole job table); the
+// kernels write the LDM axes and sizes into a separate LdmAxes array, so no
+// host round trip sits inside the pipeline and the same launch sequence
+// serves one request, a ragged batch, or a replayed hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LDM_MAX_AX 64      // maximum N_G or N_L
+#define LDM_MAX_ISO 16     // isotopologue slots
+#define LDM_MAX_RAD 12     // radix stages per sub-FFT
+#define LDM_CAP 8192       // complex elements held in LDS by one FFT workgroup
+#define LDM_FFT_T 512      // threads of an FFT workgroup
+#define LDM_FILL_T 256     // threads of a fill workgroup
+#define LDM_AX_CHUNK 8192  // lines per axis-reduction workgroup
+#define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
+#define LDM_NFOLD_MAX 100000
+
+// device status codes (LdmJob::status)
+#define LDM_ST_OK 0
+#define LDM_ST_AXIS 1      // N_G or N_L above LDM_MAX_AX
+#define LDM_ST_CAP 2       // N_G*N_L above the allocated LDM rows
+#define LDM_ST_FOLD 4      // alias folding did not converge
+#define LDM_ST_WIDTH 8     // non-positive broadening width
+
+struct LdmDb {
+  const double* nu0;
+  const float* S_ref;
+  const float* El;
+  const float* g_air;
+  const float* g_self;
+  const float* n_air;
+  const float* n_self;   // may be null
+  const float* delta;    // may be null
+  const uint8_t* iso;    // may be null
+};
+
+struct LdmJob {
+  double wmin, delta_w, wstep;   // grid w[i] = wmin + i*delta_w
+  long long nv, v;               // N_v and V = N_v + 1 (written LDM length)
+  long long L;                   // FFT length = n1*n2
+  long long i0, i1;              // window line range
+  double p_atm, x_self, x_air;   // pressure (atm), mole fractions
+  double log_r;                  // log(Tref/T)
+  double c_boltz;                // -hc_k*(1/T - 1/Tref)
+  double i_scale;                // number density * 2^-scale_log2
+  double dopp[LDM_MAX_ISO];      // sqrt((2 Na k_b T ln2)/M_iso)
+  double qratio[LDM_MAX_ISO];    // Q(Tref)/Q(T)
+  double dxG, dxL, fold_thresh;
+  double shift_margin;           // max|delta_air|*p_atm + 2*delta_w
+  float c_stim_T, c_stim_R;      // -hc_k/T, -hc_k/Tref (stimulated emission, FP32)
+  float out_scale;               // 1/(L*wstep)
+  float inv_Lws;                 // 1/(L*wstep) (frequency step of the FT grid)
+  float half_inv_ws;             // 1/(2*wstep)
+  int single_iso;                // 1: every line is iso slot 1 and the G axis comes from the window ends
+  int n1, n2;                    // four-step split
+  int nrad1, nrad2;
+  int rad1[LDM_MAX_RAD], rad2[LDM_MAX_RAD];
+  int bA, bB;                    // columns per workgroup in passes A/C and pass B
+  int m_cap, tile_w, n_tiles;
+  long long ldm_stride;          // floats per LDM row
+  float* ldm;                    // [m_cap][ldm_stride]
+  float2* ybuf;                  // [(m_cap+1)/2][L]
+  float2* bbuf;                  // [L]
+  float* out;                    // [nv]
+  long long* tiles;              // [2*n_tiles] fill line bounds
+  int* nfold;                    // [m_cap]
+  float* row_a;                  // [m_cap] (pi*wG_l)^2/(4 ln2)
+  float* row_b;                  // [m_cap] pi*wL_m
+  float* row_inv_f0;             // [m_cap] 1/F_lm(0)
+  const float2* tw1;             // exp(-2 pi i e/n1), e < n1
+  const float2* tw2;             // exp(-2 pi i e/n2), e < n2
+  const float2* twl_hi;          // W_L^(h*1024)
+  const float2* twl_lo;          // W_L^(e), e < 1024
+  // reduction targets and status, initialised by the host every run
+  unsigned long long wl_min_bits, wl_max_bits, wg_min_bits, wg_max_bits;
+  int status;
+  int pad0_;
+};
+
+// Device-computed LDM axes of one request (written by k_finalize).
+struct LdmAxes {
+  int ng, nl, m, p;                          // N_G, N_L, rows M = N_G*N_L, pairs P = ceil(M/2)
+  double laG[LDM_MAX_AX], slG[LDM_MAX_AX];   // log of G-axis nodes, 1/(la[j+1]-la[j])
+  double laL[LDM_MAX_AX], slL[LDM_MAX_AX];
+};
+
+// Per-request results copied back to the host after every run.
+struct LdmResult {
+  int ng, nl, status, pad_;
+};

@@ -1,0 +1,572 @@
+// HIP kernels of the DIT/LDM pipeline (gfx950).
+//
+//   k_axis        per-request min/max of the line widths (LDM axis limits)
+//   k_finalize    LDM axes, N_G, N_L, per-row alias-fold counts and norms
+//   k_tile_bounds line range of every fill tile (binary search on nu0)
+//   k_fill        line parameters + bit-exact np.interp indices + 2x2x2
+//                 weights, scattered into an LDS tile [M][W] and flushed once
+//   k_fft_a       four-step pass 1: N1-point DFTs over strided columns of two
+//                 LDM rows packed as one complex row, times W_L^(n2 k1)
+//   k_fft_b       four-step pass 2 fused with the lineshape multiply and the
+//                 (l, m) reduction, then the first inverse pass
+//   k_fft_c       last inverse pass, real part, 1/(L wstep) scale and crop
+//
+// Semantics follow RADIS's CPU LDM-fft path; file:line citations are to the
+// radis 0.17.1 sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ldm_fft.h"
+#include "ldm_job.h"
+
+#define LDM_C_CGS 2.99792458e10       /* radis/phys/constants.py:78 */
+
+// ---------------------------------------------------------------------------
+// FP64 arithmetic that must round exactly like NumPy: no FMA contraction.
+// ---------------------------------------------------------------------------
+
+// Position of x on the grid w[i] = wmin + i*delta (numpy.arange fill rule),
+// following numpy.interp(x, w, arange(nv)) bit for bit:
+//   x < w[0] -> 0; x >= w[nv-1] -> nv-1; w[j] == x -> j;
+//   else (1/(w[j+1]-w[j]))*(x-w[j]) + j with w[j] <= x < w[j+1].
+// radis/lbl/broadening.py:2151-2154, 2253.
+__device__ __noinline__ double grid_position(double x, double wmin, double delta, long long nv) {
+#pragma clang fp contract(off)
+  const double wlast = wmin + (double)(nv - 1) * delta;
+  if (x < wmin) return 0.0;
+  if (!(x < wlast)) return (double)(nv - 1);
+  long long j = (long long)floor((x - wmin) / delta);
+  if (j < 0) j = 0;
+  if (j > nv - 2) j = nv - 2;
+  while (j > 0 && wmin + (double)j * delta > x) --j;
+  while (j < nv - 2 && wmin + (double)(j + 1) * delta <= x) ++j;
+  const double wj = wmin + (double)j * delta;
+  if (wj == x) return (double)j;
+  const double wj1 = wmin + (double)(j + 1) * delta;
+  const double slope = 1.0 / (wj1 - wj);
+  return slope * (x - wj) + (double)j;
+}
+
+// Same rule on a small tabulated increasing axis la[0..n) with the slopes
+// sl[j] = 1/(la[j+1]-la[j]) precomputed (NumPy computes the same quotient).
+__device__ __forceinline__ double axis_position(double x, const double* la, const double* sl, int n) {
+#pragma clang fp contract(off)
+  if (x < la[0]) return 0.0;
+  if (!(x < la[n - 1])) return (double)(n - 1);
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (la[mid] <= x) lo = mid; else hi = mid;
+  }
+  if (la[lo] == x) return (double)lo;
+  return sl[lo] * (x - la[lo]) + (double)lo;
+}
+
+// Gaussian FWHM, radis/lbl/broadening.py:170-172 (x2 at 1855):
+// ((nu0 / c_CGS) * sqrt((2 Na k_b T ln2)/M)) * 2
+__device__ __forceinline__ double fwhm_gauss(double nu0, double dopp) {
+#pragma clang fp contract(off)
+  return ((nu0 / LDM_C_CGS) * dopp) * 2.0;
+}
+
+// Lorentzian FWHM, radis/lbl/broadening.py:341-383 (x2 at 1854), with
+// diluent {'air': 1-x}; (Tref/T)**n evaluated as exp(n*log(Tref/T)).
+__device__ __forceinline__ double fwhm_lorentz(const LdmJob& J, double g_air, double g_self, double n_air,
+                                               double n_self) {
+#pragma clang fp contract(off)
+  const double ra = exp(n_air * J.log_r);
+  const double rs = (n_self == n_air) ? ra : exp(n_self * J.log_r);
+  const double air = ((ra * g_air) * J.p_atm) * J.x_air;
+  const double slf = rs * ((g_self * J.p_atm) * J.x_self);
+  return (air + slf) * 2.0;
+}
+
+__device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long double_to_bits(double d) {
+  return (unsigned long long)__double_as_longlong(d);
+}
+
+// ---------------------------------------------------------------------------
+// k_axis: min/max of the Lorentzian (and, for several isotopologues, the
+// Gaussian) FWHM over each request's window; radis/lbl/broadening.py:1823-1852.
+// Positive doubles order like their bit patterns, so the reduction uses
+// unsigned 64-bit atomics.  Item = (job, first line, last line).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
+                                              LdmDb db) {
+  const int4 it = items[blockIdx.x];
+  LdmJob& J = jobs[it.x];
+  const long long a = J.i0 + (long long)it.y * LDM_AX_CHUNK;
+  long long e = a + LDM_AX_CHUNK;
+  if (e > J.i1) e = J.i1;
+  unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
+  for (long long i = a + threadIdx.x; i < e; i += blockDim.x) {
+    const double na = db.n_air[i];
+    const double ns = db.n_self ? (double)db.n_self[i] : na;
+    const double wl = fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, ns);
+    const unsigned long long bl = double_to_bits(wl);
+    lmin = bl < lmin ? bl : lmin;
+    lmax = bl > lmax ? bl : lmax;
+    if (!J.single_iso) {
+      const int iso = db.iso ? db.iso[i] : 1;
+      const unsigned long long bg = double_to_bits(fwhm_gauss(db.nu0[i], J.dopp[iso]));
+      gmin = bg < gmin ? bg : gmin;
+      gmax = bg > gmax ? bg : gmax;
+    }
+  }
+  // wave reduction
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o;
+    o = __shfl_xor(lmin, off); lmin = o < lmin ? o : lmin;
+    o = __shfl_xor(lmax, off); lmax = o > lmax ? o : lmax;
+    o = __shfl_xor(gmin, off); gmin = o < gmin ? o : gmin;
+    o = __shfl_xor(gmax, off); gmax = o > gmax ? o : gmax;
+  }
+  __shared__ unsigned long long red[4][4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][w] = lmin; red[1][w] = lmax; red[2][w] = gmin; red[3][w] = gmax; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int k = 1; k < nw; ++k) {
+      lmin = red[0][k] < lmin ? red[0][k] : lmin;
+      lmax = red[1][k] > lmax ? red[1][k] : lmax;
+      gmin = red[2][k] < gmin ? red[2][k] : gmin;
+      gmax = red[3][k] > gmax ? red[3][k] : gmax;
+    }
+    if (e > a) {
+      atomicMin(&J.wl_min_bits, lmin);
+      atomicMax(&J.wl_max_bits, lmax);
+      if (!J.single_iso) {
+        atomicMin(&J.wg_min_bits, gmin);
+        atomicMax(&J.wg_max_bits, gmax);
+      }
+    }
+  }
+}
+
+// Voigt FT at x for FWHM (wG, wL): exp(-(pi x wG)^2/(4 ln2)) * exp(-pi x wL)
+// (radis/lbl/broadening.py:252, 604, 850-853 with hwhm = fwhm/2).
+__device__ __forceinline__ double voigt_ft64(double x, double wG, double wL) {
+  const double PI = 3.141592653589793;
+  const double g = PI * x * wG;
+  return exp(-(g * g) / (4.0 * 0.6931471805599453)) * exp(-PI * x * wL);
+}
+
+// ---------------------------------------------------------------------------
+// k_finalize: one workgroup per request.  Builds the two log-spaced axes
+// exactly as _init_w_axis (radis/lbl/broadening.py:1823-1837), the per-row
+// alias-fold counts and normalisations of the FT templates (1895-1926).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_finalize(LdmJob* __restrict__ jobs, LdmAxes* __restrict__ axes,
+                                                 LdmResult* __restrict__ res, LdmDb db) {
+  LdmJob& J = jobs[blockIdx.x];
+  LdmAxes& A = axes[blockIdx.x];
+  __shared__ double wgax[LDM_MAX_AX], wlax[LDM_MAX_AX];
+  __shared__ int s_ng, s_nl, s_status;
+  if (threadIdx.x == 0) {
+    int status = 0, ng = 0, nl = 0;
+    if (J.i1 > J.i0) {
+      double wg_min, wg_max;
+      if (J.single_iso) {
+        wg_min = fwhm_gauss(db.nu0[J.i0], J.dopp[1]);
+        wg_max = fwhm_gauss(db.nu0[J.i1 - 1], J.dopp[1]);
+      } else {
+        wg_min = bits_to_double(J.wg_min_bits);
+        wg_max = bits_to_double(J.wg_max_bits);
+      }
+      const double wl_min = bits_to_double(J.wl_min_bits);
+      const double wl_max = bits_to_double(J.wl_max_bits);
+      if (!(wg_min > 0.0) || !(wl_min > 0.0)) {
+        status |= LDM_ST_WIDTH;
+      } else {
+        const double gmx = wg_max + 1e-4, lmx = wl_max + 1e-4;
+        ng = (int)ceil((log(gmx) - log(wg_min)) / J.dxG) + 1;
+        nl = (int)ceil((log(lmx) - log(wl_min)) / J.dxL) + 1;
+        if (ng > LDM_MAX_AX || nl > LDM_MAX_AX || ng < 2 || nl < 2) {
+          status |= LDM_ST_AXIS;
+        } else {
+          for (int i = 0; i < ng; ++i) {
+            wgax[i] = wg_min * exp(J.dxG * (double)i);
+            A.laG[i] = log(wgax[i]);
+          }
+          for (int i = 0; i < nl; ++i) {
+            wlax[i] = wl_min * exp(J.dxL * (double)i);
+            A.laL[i] = log(wlax[i]);
+          }
+          for (int i = 0; i + 1 < ng; ++i) A.slG[i] = 1.0 / (A.laG[i + 1] - A.laG[i]);
+          for (int i = 0; i + 1 < nl; ++i) A.slL[i] = 1.0 / (A.laL[i + 1] - A.laL[i]);
+          if (ng * nl > J.m_cap) status |= LDM_ST_CAP;
+        }
+      }
+    }
+    if (status) { ng = 0; nl = 0; }
+    A.ng = ng;
+    A.nl = nl;
+    A.m = ng * nl;
+    A.p = (ng * nl + 1) / 2;
+    s_ng = ng; s_nl = nl; s_status = status;
+  }
+  __syncthreads();
+  const int ng = s_ng, nl = s_nl;
+  int status = 0;
+  const double PI = 3.141592653589793;
+  const double h = 0.5 / J.wstep;                      // n/(2 wstep) step
+  const double xrev0 = (double)(J.L / 2) * (1.0 / ((double)J.L * J.wstep));  // rfftfreq(L, wstep)[-1]
+  for (int r = threadIdx.x; r < ng * nl; r += blockDim.x) {
+    const int l = r / nl, m = r - (r / nl) * nl;
+    const double wG = wgax[l], wL = wlax[m];
+    int n = 1;
+    while (n < LDM_NFOLD_MAX && voigt_ft64(n * h, wG, wL) >= J.fold_thresh) ++n;
+    if (n >= LDM_NFOLD_MAX) status |= LDM_ST_FOLD;
+    const int nf = n - 1;
+    double f0 = 1.0;
+    for (int q = 1; q <= nf; ++q) f0 += voigt_ft64(q * h + ((q & 1) ? xrev0 : 0.0), wG, wL);
+    J.nfold[r] = nf;
+    J.row_a[r] = (float)((PI * wG) * (PI * wG) / (4.0 * 0.6931471805599453));
+    J.row_b[r] = (float)(PI * wL);
+    J.row_inv_f0[r] = (float)(1.0 / f0);
+  }
+  if (status) atomicOr(&J.status, status);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    J.status |= s_status;
+    res[blockIdx.x].ng = A.ng;
+    res[blockIdx.x].nl = A.nl;
+    res[blockIdx.x].status = J.status;
+  }
+}
+
+// first index in [lo, hi) with nu0[i] >= v
+__device__ __forceinline__ long long lower_bound_nu(const double* __restrict__ nu0, long long lo, long long hi,
+                                                    double v) {
+  while (lo < hi) {
+    const long long mid = lo + ((hi - lo) >> 1);
+    if (nu0[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// k_tile_bounds: for every fill tile [k_lo, k_hi) the line range that can
+// put a corner inside it: k0 in [k_lo-1, k_hi-1].  A line's np.interp
+// position is monotone in its shifted centre, so a search on nu0 widened by
+// the largest pressure shift (plus two grid steps for rounding) is a
+// superset; k_fill filters each line by its exact k0.  Item = (job, tile).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
+                                                     int n_items, LdmDb db) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_items) return;
+  const int4 it = items[g];
+  LdmJob& J = jobs[it.x];
+  const long long t = it.y;
+  const long long k_lo = t * J.tile_w;
+  long long k_hi = k_lo + J.tile_w;
+  if (k_hi > J.v) k_hi = J.v;
+  long long lo, hi;
+  if (k_lo <= 1) lo = J.i0;
+  else lo = lower_bound_nu(db.nu0, J.i0, J.i1, J.wmin + (double)(k_lo - 1) * J.delta_w - J.shift_margin);
+  if (k_hi >= J.nv) hi = J.i1;
+  else hi = lower_bound_nu(db.nu0, J.i0, J.i1, J.wmin + (double)k_hi * J.delta_w + J.shift_margin);
+  if (hi < lo) hi = lo;
+  J.tiles[2 * t] = lo;
+  J.tiles[2 * t + 1] = hi;
+}
+
+// Everything the fill needs from one line, in FP64 where NumPy parity of
+// the integer indices requires it:
+//   shifted centre        radis/lbl/base.py:2351-2360
+//   np.interp indices     radis/lbl/broadening.py:2151-2154, 2253-2263
+//   FWHM_G, FWHM_L        radis/lbl/broadening.py:170-172, 341-383
+//   S(T)                  radis/lbl/base.py:2664-2674 (Boltzmann and Q ratio FP64)
+// I = S(T) * number density * 2^-scale in FP32.
+struct LineTerms {
+  long long k0;
+  int l0, m0;
+  float tv, tg, tl, I;
+};
+
+__device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, long long i, const double* laG,
+                                           const double* slG, const double* laL, const double* slL, int ng, int nl,
+                                           LineTerms& t) {
+  const double nu0 = db.nu0[i];
+  double nu = nu0;
+  if (db.delta) {
+#pragma clang fp contract(off)
+    nu = nu0 + ((double)db.delta[i] * J.p_atm);
+  }
+  const double pos = grid_position(nu, J.wmin, J.delta_w, J.nv);
+  t.k0 = (long long)pos;
+  t.tv = (float)(pos - (double)t.k0);
+  const int iso = db.iso ? db.iso[i] : 1;
+  const double na = db.n_air[i];
+  const double ns = db.n_self ? (double)db.n_self[i] : na;
+  const double wl = fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, ns);
+  const double wg = fwhm_gauss(nu0, J.dopp[iso]);
+  const double pg = axis_position(log(wg), laG, slG, ng);
+  const double pl = axis_position(log(wl), laL, slL, nl);
+  t.l0 = (int)pg;
+  t.m0 = (int)pl;
+  t.tg = (float)(pg - (double)t.l0);
+  t.tl = (float)(pl - (double)t.m0);
+  if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
+  if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
+  const double s64 = (double)db.S_ref[i] * J.qratio[iso] * exp(J.c_boltz * (double)db.El[i]) * J.i_scale;
+  const float nuf = (float)nu0;
+  const float stim = expm1f(J.c_stim_T * nuf) / expm1f(J.c_stim_R * nuf);
+  t.I = (float)s64 * stim;
+}
+
+// ---------------------------------------------------------------------------
+// k_fill: one workgroup per (request, tile of W grid points).  Every line
+// whose 2x2x2 stencil touches the tile computes, in FP64:
+//   shifted centre        radis/lbl/base.py:2351-2360
+//   S(T) (Boltzmann, Q)   radis/lbl/base.py:2664-2674
+//   FWHM_G, FWHM_L        radis/lbl/broadening.py:170-172, 341-383
+//   np.interp indices     radis/lbl/broadening.py:2151-2154, 2253-2263
+// then adds the simple weights (2305-2318) of its corners that fall in the
+// tile into LDS with ds_add_f32.  The tile is written to HBM once, so the
+// LDM needs no clearing and no global atomics.  Row index r = l*N_L + m.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
+                                                     const LdmAxes* __restrict__ axes,
+                                                     const int4* __restrict__ items, LdmDb db) {
+  extern __shared__ float acc[];
+  __shared__ double s_laG[LDM_MAX_AX], s_slG[LDM_MAX_AX], s_laL[LDM_MAX_AX], s_slL[LDM_MAX_AX];
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  const LdmAxes& A = axes[it.x];
+  const int M = A.m;
+  if (M == 0 || J.status) return;
+  const int ng = A.ng, nl = A.nl;
+  const int W = J.tile_w;
+  const long long k_lo = (long long)it.y * W;
+  long long k_hi = k_lo + W;
+  if (k_hi > J.v) k_hi = J.v;
+  for (int e = threadIdx.x; e < M * W; e += blockDim.x) acc[e] = 0.0f;
+  for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
+    s_laG[e] = A.laG[e]; s_slG[e] = A.slG[e];
+    s_laL[e] = A.laL[e]; s_slL[e] = A.slL[e];
+  }
+  __syncthreads();
+  const long long lo = J.tiles[2 * it.y], hi = J.tiles[2 * it.y + 1];
+  for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    LineTerms t;
+    line_terms(J, db, i, s_laG, s_slG, s_laL, s_slL, ng, nl, t);
+    const long long k0 = t.k0;
+    if (k0 < k_lo - 1 || k0 > k_hi - 1) continue;
+    const int l0 = t.l0, m0 = t.m0;
+    const float iv[2] = {t.I * (1.0f - t.tv), t.I * t.tv};
+    const float ag[2] = {1.0f - t.tg, t.tg};
+    const float al[2] = {1.0f - t.tl, t.tl};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const long long k = k0 + a;
+      if (k < k_lo || k >= k_hi) continue;
+      const int c = (int)(k - k_lo);
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const int r = (l0 + b) * nl + (m0 + d);
+          atomicAdd(&acc[r * W + c], iv[a] * ag[b] * al[d]);
+        }
+    }
+  }
+  __syncthreads();
+  const int wlen = (int)(k_hi - k_lo);
+  for (int e = threadIdx.x; e < M * W; e += blockDim.x) {
+    const int r = e / W, c = e - (e / W) * W;
+    if (c < wlen) J.ldm[(long long)r * J.ldm_stride + k_lo + c] = acc[e];
+  }
+}
+
+// four-step twiddle W_L^e (forward sign)
+__device__ __forceinline__ float2 twiddle_L(const LdmJob& J, long long e) {
+  const float2 h = J.twl_hi[e >> LDM_TW_LO_BITS];
+  const float2 l = J.twl_lo[e & ((1 << LDM_TW_LO_BITS) - 1)];
+  return cmul(h, l);
+}
+
+// ---------------------------------------------------------------------------
+// k_fft_a: item = (job, pair p, column block cb).  Packs LDM rows 2p and
+// 2p+1 as z = x_2p + i x_2p+1 (rows are zero beyond V), runs the N1-point
+// DFTs of columns n2 in [cb*B, cb*B+B) and writes
+//   Y_p[k1][n2] = W_L^(n2 k1) * sum_n1 z[n2 + N2 n1] W_N1^(n1 k1).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(LDM_FFT_T) void k_fft_a(const LdmJob* __restrict__ jobs,
+                                                     const LdmAxes* __restrict__ axes,
+                                                     const int4* __restrict__ items) {
+  __shared__ float2 buf[LDM_CAP];
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  const int p = it.y;
+  if (J.status || p >= axes[it.x].p) return;
+  const int N1 = J.n1, N2 = J.n2, bs = J.bA, B = 1 << bs;
+  const long long n2_0 = (long long)it.z * B;
+  const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
+  const bool has_b = (2 * p + 1) < axes[it.x].m;
+  const float* rb = ra + J.ldm_stride;
+  const long long V = J.v;
+  for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
+    const int n1 = e >> bs, b = e & (B - 1);
+    const long long n = n2_0 + b + (long long)N2 * n1;
+    float xa = 0.0f, xb = 0.0f;
+    if (n < V) {
+      xa = ra[n];
+      if (has_b) xb = rb[n];
+    }
+    buf[e] = make_float2(xa, xb);
+  }
+  __syncthreads();
+  lds_fft<true>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
+  float2* Y = J.ybuf + (long long)p * J.L;
+  for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
+    const int k1 = e >> bs, b = e & (B - 1);
+    const long long n2 = n2_0 + b;
+    const long long ex = (n2 * k1) % J.L;
+    Y[(long long)k1 * N2 + n2] = cmul(buf[e], twiddle_L(J, ex));
+  }
+}
+
+// Folded, normalised FT lineshape of LDM row r at bin k (|k| folded to
+// [0, L/2]); radis/lbl/broadening.py:1895-1926.
+__device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, long long kk) {
+  const float a = J.row_a[r], bco = J.row_b[r];
+  const float x = (float)kk * J.inv_Lws;
+  float v = __expf(-(a * x * x) - bco * x);
+  const int nf = J.nfold[r];
+  if (nf > 0) {
+    const float xr = (float)(J.L / 2 - kk) * J.inv_Lws;
+    for (int n = 1; n <= nf; ++n) {
+      const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xr) : ((float)n * J.half_inv_ws + x);
+      v += __expf(-(a * xa * xa) - bco * xa);
+    }
+  }
+  return v * J.row_inv_f0[r];
+}
+
+// ---------------------------------------------------------------------------
+// k_fft_b: item = (job, block of B2 rows k1).  For every pair p:
+//   Z_p[k1 + N1 k2] = sum_n2 Y_p[k1][n2] W_N2^(n2 k2)
+//   A[k] += Z_p[k] * (F_2p[k] - i F_2p+1[k])
+// and then B[k1][n2'] = W_L^(-n2' k1) * sum_k2 A[k1 + N1 k2] W_N2^(-n2' k2).
+// With real LDM rows x_a, x_b and real symmetric F, Re(IFFT(A)) equals
+// IFFT(X_a F_a + X_b F_b): the pair packing needs no real-FFT unpacking.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(LDM_FFT_T) void k_fft_b(const LdmJob* __restrict__ jobs,
+                                                     const LdmAxes* __restrict__ axes,
+                                                     const int4* __restrict__ items) {
+  __shared__ float2 buf[LDM_CAP];
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  if (J.status) return;
+  const int N1 = J.n1, N2 = J.n2, bs = J.bB, B = 1 << bs;
+  const int k1_0 = it.y * B;
+  const long long L = J.L;
+  const int P = axes[it.x].p, M = axes[it.x].m;
+  const int total = N2 << bs;
+  constexpr int NACC = LDM_CAP / LDM_FFT_T;
+  float2 acc[NACC];
+#pragma unroll
+  for (int s = 0; s < NACC; ++s) acc[s] = make_float2(0.0f, 0.0f);
+  for (int p = 0; p < P; ++p) {
+    const float2* Y = J.ybuf + (long long)p * L;
+    // coalesced: n2 fastest in global memory, transposed into [n2][b] in LDS
+    for (int e = threadIdx.x; e < total; e += LDM_FFT_T) {
+      const int b = e / N2, n2 = e - (e / N2) * N2;
+      buf[(n2 << bs) + b] = Y[(long long)(k1_0 + b) * N2 + n2];
+    }
+    __syncthreads();
+    lds_fft<true>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
+    const int ra = 2 * p, rb = 2 * p + 1;
+#pragma unroll
+    for (int s = 0; s < NACC; ++s) {
+      const int e = threadIdx.x + s * LDM_FFT_T;
+      if (e < total) {
+        const int k2 = e >> bs, b = e & (B - 1);
+        const long long k = (long long)(k1_0 + b) + (long long)N1 * k2;
+        const long long kk = (k <= L - k) ? k : (L - k);
+        const float fa = lineshape_ft(J, ra, kk);
+        const float fb = (rb < M) ? lineshape_ft(J, rb, kk) : 0.0f;
+        const float2 z = buf[e];
+        acc[s].x += z.x * fa + z.y * fb;
+        acc[s].y += z.y * fa - z.x * fb;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int s = 0; s < NACC; ++s) {
+    const int e = threadIdx.x + s * LDM_FFT_T;
+    if (e < total) buf[e] = acc[s];
+  }
+  __syncthreads();
+  lds_fft<false>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
+  for (int e = threadIdx.x; e < total; e += LDM_FFT_T) {
+    const int b = e / N2, n2 = e - (e / N2) * N2;
+    const int k1 = k1_0 + b;
+    const long long ex = ((long long)n2 * k1) % L;
+    J.bbuf[(long long)k1 * N2 + n2] = cmulc(buf[(n2 << bs) + b], twiddle_L(J, ex));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_fft_c: item = (job, column block cb).  y[n2 + N2 n1] =
+// sum_k1 B[k1][n2] W_N1^(-n1 k1); abscoeff[n] = Re(y[n]) / (L wstep) for
+// n < N_v (radis/lbl/broadening.py:2529-2530 crop and /wstep).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(LDM_FFT_T) void k_fft_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items) {
+  __shared__ float2 buf[LDM_CAP];
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  const int N1 = J.n1, N2 = J.n2, bs = J.bA, B = 1 << bs;
+  const long long n2_0 = (long long)it.z * B;
+  if (J.status) {
+    for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
+      const long long n = n2_0 + (e & (B - 1)) + (long long)N2 * (e >> bs);
+      if (n < J.nv) J.out[n] = 0.0f;
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
+    const int k1 = e >> bs, b = e & (B - 1);
+    buf[e] = J.bbuf[(long long)k1 * N2 + n2_0 + b];
+  }
+  __syncthreads();
+  lds_fft<false>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
+  const float sc = J.out_scale;
+  for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
+    const int n1 = e >> bs, b = e & (B - 1);
+    const long long n = n2_0 + b + (long long)N2 * n1;
+    if (n < J.nv) J.out[n] = buf[e].x * sc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_line_terms: the per-line values k_fill uses, for every line of one
+// request's window, written out for validation against NumPy.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_line_terms(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
+                                                    int job, LdmDb db, long long* __restrict__ k0, int* __restrict__ l0,
+                                                    int* __restrict__ m0, float* __restrict__ tv, float* __restrict__ tg,
+                                                    float* __restrict__ tl, float* __restrict__ I) {
+  __shared__ double s_laG[LDM_MAX_AX], s_slG[LDM_MAX_AX], s_laL[LDM_MAX_AX], s_slL[LDM_MAX_AX];
+  const LdmJob& J = jobs[job];
+  const LdmAxes& A = axes[job];
+  for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
+    s_laG[e] = A.laG[e]; s_slG[e] = A.slG[e];
+    s_laL[e] = A.laL[e]; s_slL[e] = A.slL[e];
+  }
+  __syncthreads();
+  if (A.m == 0 || J.status) return;
+  const long long n = J.i1 - J.i0;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    LineTerms t;
+    line_terms(J, db, J.i0 + q, s_laG, s_slG, s_laL, s_slL, A.ng, A.nl, t);
+    k0[q] = t.k0; l0[q] = t.l0; m0[q] = t.m0;
+    tv[q] = t.tv; tg[q] = t.tg; tl[q] = t.tl; I[q] = t.I;
+  }
+}
