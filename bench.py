@@ -11,6 +11,8 @@ Prints one JSON line (rank 0).  Other workloads:
   --workload co_5k | co2_5m | h2o_100m      single spectra (lines/s)
   --workload batch_256                      256 requests in one launch set
                                             (spectra/s), hipGraph replay
+  --workload serve                          Poisson 200 req/s over two
+                                            resident lists (spectra/s, p50/p99)
 """
 import argparse
 import json
@@ -36,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-lines", type=int, default=0, help="lines in the CPU baseline sample (0 = auto)")
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson request stream")
+    ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -198,13 +203,142 @@ def bench_single(args):
     return out
 
 
+def copy_reference(mib):
+    """Plain device copy of known size: the reference that FETCH_SIZE and
+    WRITE_SIZE are scaled by when HBM traffic per launch is derived."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_reference_bytes_each_way": n * 4, "copies": 5}), flush=True)
+
+
+def bench_batch(args):
+    """256 requests over the 5 M-line list in one launch set replayed as a
+    hipGraph, against 256 sequential single-request runs."""
+    import torch
+
+    from radis_amd import engine, workloads
+
+    lines = workloads.co2_like(args.lines or 5_000_000)
+    conds = workloads.random_requests(256, 2000.0, 2500.0, seed=1003, wsteps=(0.001,))
+    dev = engine.Device(0)
+    db = engine.LineDatabase(dev, lines)
+    reqs = [engine.request_from_conditions(db, c) for c in conds]
+    batch = engine.SpectrumBatch(db, reqs)
+    n_lines = sum(int(batch.infos[i].line_end - batch.infos[i].line_begin) for i in range(batch.n))
+    stream = torch.cuda.Stream()
+    # per-stage device times from eager runs
+    outs = batch.alloc_outputs()
+    batch.enable_timing(True)
+    for _ in range(3):
+        batch.run(outs, stream)
+    stages = batch.stage_ms()
+    batch.enable_timing(False)
+    gouts = batch.capture(stream=stream)
+    for _ in range(args.warmup):
+        batch.replay(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.replay(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = [batch.info(i).status for i in range(batch.n)]
+    if any(st):
+        raise RuntimeError(f"device status {st}")
+    # sequential plain calls
+    singles = [engine.SpectrumBatch(db, [r]) for r in reqs]
+    souts = [b.alloc_outputs() for b in singles]
+    for b, o in zip(singles, souts):
+        b.run(o, stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for b, o in zip(singles, souts):
+        b.run(o, stream)
+    torch.cuda.synchronize()
+    dt_seq = time.perf_counter() - t1
+    mx = max(float(torch.max(torch.abs(a - c)).item() / max(torch.max(torch.abs(c)).item(), 1e-300))
+             for a, c in zip(gouts[:16], [o[0] for o in souts[:16]]))
+    ws = batch.workspace_bytes()
+    for b in singles:
+        b.close()
+    batch.close()
+    db.close()
+    dev.close()
+    value = 256 * args.steps / dt
+    return {
+        "metric": METRIC, "value": round(value, 2), "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 indices, Boltzmann, Q ratio)",
+        "data": "synthetic",
+        "config": {"workload": "batch_256", "lines": len(lines), "requests": 256, "window_lines_total": n_lines,
+                   "wstep": 0.001, "graph": True, "workspace_bytes": ws, "parallelism": "single-gpu"},
+        "lines_per_s": round(n_lines * args.steps / dt, 1),
+        "sequential_256": {"spectra_per_s": round(256 / dt_seq, 2), "ms": round(dt_seq * 1e3, 3),
+                           "speedup_batched": round((dt_seq / (dt / args.steps)), 2)},
+        "batched_vs_sequential_max_rel_diff": mx,
+        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+    }
+
+
+def bench_serve(args):
+    """Poisson arrivals at 200 req/s against two resident lists."""
+    from radis_amd import engine, serve, workloads
+
+    dev = engine.Device(0)
+    dbs = {"co2": engine.LineDatabase(dev, workloads.co2_like(5_000_000)),
+           "h2o": engine.LineDatabase(dev, workloads.h2o_like(20_000_000, seed=1005))}
+    dur = args.serve_seconds
+    t = serve.poisson_arrivals(200.0, dur, seed=1004)
+    ca = workloads.random_requests(len(t), 2000.0, 2500.0, seed=1006, wsteps=(0.001, 0.002, 0.005))
+    cb = workloads.random_requests(len(t), 1000.0, 4000.0, seed=1007, wsteps=(0.001, 0.002, 0.005))
+    arrivals = [(ti, "co2", ca[i]) if i % 2 == 0 else (ti, "h2o", cb[i]) for i, ti in enumerate(t)]
+    srv = serve.SpectrumServer(dev, dbs, budget_bytes=96 << 30, max_batch=32, n_streams=4)
+    # warm-up: a few requests through the same path
+    srv.run_stream([(0.0, a[1], a[2]) for a in arrivals[:8]], timeout=120)
+    srv.done.clear()
+    t0 = time.perf_counter()
+    done = srv.run_stream(arrivals, timeout=dur * 5 + 60)
+    wall = time.perf_counter() - t0
+    errors = [r for r in done if r.error]
+    lat = serve.latency_summary(done)
+    n_lines = sum(r.n_lines for r in done)
+    span = max(r.t_done for r in done) - min(r.t_arrival for r in done)
+    for d in dbs.values():
+        d.close()
+    dev.close()
+    return {
+        "metric": METRIC, "value": round(len(done) / span, 2), "unit": "spectra/s", "n_gpus": 1,
+        "steps": len(done), "warmup": 8, "ms_per_step": round(span / max(len(done), 1) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 indices, Boltzmann, Q ratio)", "data": "synthetic",
+        "config": {"workload": "serve", "arrival_rate_per_s": 200, "duration_s": dur,
+                   "dbs": {"co2": 5_000_000, "h2o": 20_000_000}, "wsteps": [0.001, 0.002, 0.005],
+                   "max_batch": 32, "streams": 4, "parallelism": "single-gpu"},
+        "latency": lat, "errors": len(errors), "lines_per_s": round(n_lines / span, 1), "wall_s": round(wall, 2),
+    }
+
+
 def main():
     args = parse()
+    if args.copy_ref:
+        copy_reference(args.copy_ref)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(every box here has one MI355X)", file=sys.stderr)
         sys.exit(2)
-    out = bench_single(args)
+    if args.workload == "batch_256":
+        out = bench_batch(args)
+    elif args.workload == "serve":
+        out = bench_serve(args)
+    else:
+        out = bench_single(args)
     print(json.dumps(out), flush=True)
 
 

@@ -85,7 +85,12 @@ typedef struct {
                                  is wmin + i*((wmin+wstep)-wmin), i < n_v (gpuApp's Nv) */
     double line_lo, line_hi;  /* if line_hi > line_lo: lines with line_lo<=nu0<=line_hi,
                                  otherwise wmin<=nu0<=wmax (lbl/loader.py:2707)          */
+    int32_t flags;            /* RADIS_LDM_FP64_WIDTHS: log-width axis positions and
+                                 fractions always in FP64 (default: FP32 with an exact
+                                 FP64 recomputation next to axis nodes)                  */
 } radis_ldm_request;
+
+#define RADIS_LDM_FP64_WIDTHS 1
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {
@@ -117,6 +122,12 @@ int radis_ldm_db_size(const radis_ldm_db *db, int64_t *n_lines);
 /* --- planning helpers ----------------------------------------------------- */
 /* Grid length of arange(wmin, wmax+wstep, wstep) and the planner's FFT length. */
 int radis_ldm_plan_grid(double wmin, double wmax, double wstep, int64_t *n_v, int64_t *fft_len);
+
+/* Device workspace a request would need in a batch of its own (bytes), and
+ * its planned sizes, without allocating anything: admission control by HBM
+ * budget in the serving loop. */
+int radis_ldm_request_bytes(const radis_ldm_db *db, const radis_ldm_request *req, int64_t *bytes,
+                            radis_ldm_request_info *info);
 
 /* --- batches -------------------------------------------------------------- */
 /* Plan n requests over db and allocate their workspace.  Replaces gpuApp's

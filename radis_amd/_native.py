@@ -73,7 +73,11 @@ class Request(ctypes.Structure):
         ("n_v", c_int64),
         ("line_lo", c_double),
         ("line_hi", c_double),
+        ("flags", c_int32),
     ]
+
+
+FP64_WIDTHS = 1
 
 
 class RequestInfo(ctypes.Structure):
@@ -104,6 +108,7 @@ _PROTOTYPES = {
     "radis_ldm_db_destroy": (None, [c_void_p]),
     "radis_ldm_db_size": (c_int, [c_void_p, POINTER(c_int64)]),
     "radis_ldm_plan_grid": (c_int, [c_double, c_double, c_double, POINTER(c_int64), POINTER(c_int64)]),
+    "radis_ldm_request_bytes": (c_int, [c_void_p, POINTER(Request), POINTER(c_int64), POINTER(RequestInfo)]),
     "radis_ldm_batch_create": (c_int, [c_void_p, c_void_p, c_int, POINTER(Request), POINTER(c_void_p)]),
     "radis_ldm_batch_set_request": (c_int, [c_void_p, c_int, POINTER(Request)]),
     "radis_ldm_batch_run": (c_int, [c_void_p, POINTER(c_void_p), c_void_p]),

@@ -75,9 +75,15 @@ int floor_log2(long long x) {
 struct radis_ldm_ctx {
   int device = 0;
   int64_t budget = 0;
-  int64_t in_use = 0;
+  int64_t in_use = 0;   // bytes of live line lists and batch workspaces
   int refs = 0;
   bool closed = false;
+  // Released device / pinned-host blocks kept for reuse, by size: batch
+  // set-up and tear-down in a serving loop then never call hipMalloc or
+  // hipFree (which can synchronise the device).
+  std::multimap<size_t, void*> dev_cache, host_cache;
+  size_t dev_cached = 0, host_cached = 0;
+  ~radis_ldm_ctx();
 };
 
 struct radis_ldm_db {
@@ -103,6 +109,47 @@ namespace {
 void ctx_release(radis_ldm_ctx* ctx) {
   if (--ctx->refs == 0 && ctx->closed) delete ctx;
 }
+
+// Round up to the cache granularity (256 B below 1 MiB, 1 MiB above).
+size_t cache_round(size_t n) {
+  return n < (1u << 20) ? align_up(std::max<size_t>(n, 256), 256) : align_up(n, (size_t)1 << 20);
+}
+
+// Take a cached block of at least n bytes (and at most 2n) or allocate one.
+void* cache_take(std::multimap<size_t, void*>& cache, size_t& cached, size_t n, bool host) {
+  n = cache_round(n);
+  auto it = cache.lower_bound(n);
+  if (it != cache.end() && it->first <= 2 * n) {
+    void* p = it->second;
+    cached -= it->first;
+    cache.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = host ? hipHostMalloc(&p, n) : hipMalloc(&p, n);
+  if (e != hipSuccess && !cache.empty()) {
+    // release the cache and retry once
+    for (auto& kv : cache) (void)(host ? hipHostFree(kv.second) : hipFree(kv.second));
+    cache.clear();
+    cached = 0;
+    e = host ? hipHostMalloc(&p, n) : hipMalloc(&p, n);
+  }
+  return e == hipSuccess ? p : nullptr;
+}
+
+// Block sizes are tracked by the owners; the cache keys blocks by the size
+// they were taken with (rounded the same way).
+void cache_give(std::multimap<size_t, void*>& cache, size_t& cached, void* p, size_t n) {
+  if (!p) return;
+  n = cache_round(n);
+  cache.emplace(n, p);
+  cached += n;
+}
+
+void* dev_alloc(radis_ldm_ctx* c, size_t n) { return cache_take(c->dev_cache, c->dev_cached, n, false); }
+void dev_release(radis_ldm_ctx* c, void* p, size_t n) { cache_give(c->dev_cache, c->dev_cached, p, n); }
+void* host_alloc(radis_ldm_ctx* c, size_t n) { return cache_take(c->host_cache, c->host_cached, n, true); }
+void host_release(radis_ldm_ctx* c, void* p, size_t n) { cache_give(c->host_cache, c->host_cached, p, n); }
 
 void db_free(radis_ldm_db* db) {
   if (db->mem) {
@@ -159,7 +206,9 @@ struct radis_ldm_batch {
   void* twmem = nullptr;
   size_t tw_bytes = 0;
   int4* items_d = nullptr;
+  size_t items_bytes = 0;
   int n_axis = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
+  bool small_b = true;  // pass B with LDM_CAP_B tiles (every n2 <= LDM_CAP_B)
   size_t fill_lds = 0;
   std::vector<float*> outs;        // outputs of the last eager run
   std::vector<float*> graph_outs;  // outputs bound to the captured graph
@@ -202,43 +251,55 @@ bool radices_of(int n, std::vector<int>& r) {
   return n == 1 && (int)r.size() <= LDM_MAX_RAD;
 }
 
-int plan_fft(int64_t nv, int64_t want, FftPlan& out) {
+// Search L = n1*n2 >= 2*N_v (or == want) with 16 | n1 <= 1024 and
+// 16 | n2 <= n2_max, 7-smooth cofactors; smallest L, then n1 closest to 512.
+int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2) {
   static const std::vector<int> c1 = smooth16_list(N1_MAX);
   static const std::vector<int> c2 = smooth16_list(N2_MAX);
-  const int64_t need = 2 * nv;
   int64_t bestL = -1;
   double bestScore = 1e300;
-  int b1 = 0, b2 = 0;
   for (int n1 : c1) {
-    int64_t n2min = (need + n1 - 1) / n1;
+    int n2 = 0;
     if (want > 0) {
       if (want % n1) continue;
-      int64_t n2 = want / n1;
-      if (!std::binary_search(c2.begin(), c2.end(), (int)std::min<int64_t>(n2, 1 << 30))) continue;
-      double score = std::fabs(std::log2((double)n1) - 9.0);
-      if (score < bestScore) { bestScore = score; bestL = want; b1 = n1; b2 = (int)n2; }
-      continue;
+      const int64_t q = want / n1;
+      if (q > n2_max || !std::binary_search(c2.begin(), c2.end(), (int)q)) continue;
+      n2 = (int)q;
+    } else {
+      const int64_t n2min = (need + n1 - 1) / n1;
+      if (n2min > n2_max) continue;
+      auto itn = std::lower_bound(c2.begin(), c2.end(), (int)n2min);
+      if (itn == c2.end() || *itn > n2_max) continue;
+      n2 = *itn;
     }
-    auto itn = std::lower_bound(c2.begin(), c2.end(), (int)std::min<int64_t>(n2min, 1 << 30));
-    if (itn == c2.end()) continue;
-    int64_t L = (int64_t)n1 * *itn;
-    double score = std::fabs(std::log2((double)n1) - 9.0);
+    const int64_t L = (int64_t)n1 * n2;
+    const double score = std::fabs(std::log2((double)n1) - 9.0);
     if (bestL < 0 || L < bestL || (L == bestL && score < bestScore)) {
-      bestL = L; bestScore = score; b1 = n1; b2 = *itn;
+      bestL = L; bestScore = score; b1 = n1; b2 = n2;
     }
   }
-  if (bestL < 0) {
+  return bestL;
+}
+
+int plan_fft(int64_t nv, int64_t want, FftPlan& out) {
+  const int64_t need = 2 * nv;
+  if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);
+  int a1 = 0, a2 = 0, s1 = 0, s2 = 0;
+  const int64_t any = search_fft(need, want, N2_MAX, a1, a2);
+  const int64_t small = search_fft(need, want, LDM_CAP_B, s1, s2);  // pass B with many small tiles
+  if (any < 0) {
     if (want > 0)
       return fail(RADIS_LDM_ERANGE, "fft_len %lld is not n1*n2 with 16|n1<=%d, 16|n2<=%d and 7-smooth cofactors",
                   (long long)want, N1_MAX, N2_MAX);
     return fail(RADIS_LDM_ERANGE, "grid of %lld points needs an FFT above %lld points", (long long)nv,
                 (long long)N1_MAX * N2_MAX);
   }
-  if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);
-  out.L = bestL;
-  out.n1 = b1;
-  out.n2 = b2;
-  if (!radices_of(b1, out.r1) || !radices_of(b2, out.r2)) return fail(RADIS_LDM_ERANGE, "radix factorisation failed");
+  const bool use_small = small > 0 && (double)small <= 1.03 * (double)any;
+  out.L = use_small ? small : any;
+  out.n1 = use_small ? s1 : a1;
+  out.n2 = use_small ? s2 : a2;
+  if (!radices_of(out.n1, out.r1) || !radices_of(out.n2, out.r2))
+    return fail(RADIS_LDM_ERANGE, "radix factorisation failed");
   return 0;
 }
 
@@ -332,7 +393,7 @@ int bound_request(const radis_ldm_db& db, Req& R) {
 
 int choose_tile(int m_cap) {
   int w = MAX_TILE_W;
-  while (w > MIN_TILE_W && (size_t)m_cap * w * 4 > (size_t)FILL_LDS_TARGET) w >>= 1;
+  while (w > MIN_TILE_W && (size_t)m_cap * (w + 1) * 4 > (size_t)FILL_LDS_TARGET) w >>= 1;
   return w;
 }
 
@@ -362,6 +423,11 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.dxL = r.dxL;
   J.fold_thresh = r.folding_thresh;
   J.shift_margin = (db.has_delta ? db.max_abs_delta * J.p_atm : 0.0) + 2 * R.delta_w;
+  J.log_r_f = (float)J.log_r;
+  J.pa_air_f = (float)(J.p_atm * J.x_air);
+  J.pa_self_f = (float)(J.p_atm * J.x_self);
+  for (int s = 0; s < LDM_MAX_ISO; ++s) J.lgc_f[s] = J.dopp[s] > 0 ? (float)std::log(2 * J.dopp[s] / 2.99792458e10) : 0.0f;
+  J.fp64_widths = (r.flags & RADIS_LDM_FP64_WIDTHS) ? 1 : 0;
   J.c_stim_T = (float)(-HC_K / r.Tgas);
   J.c_stim_R = (float)(-HC_K / r.Tref);
   J.out_scale = (float)(1.0 / ((double)R.plan.L * r.wstep));
@@ -404,9 +470,9 @@ void free_layout(radis_ldm_batch* b) {
   if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
   b->graph_valid = false;
-  if (b->work) { (void)hipFree(b->work); b->ctx->in_use -= (int64_t)b->work_bytes; b->work = nullptr; b->work_bytes = 0; }
-  if (b->twmem) { (void)hipFree(b->twmem); b->ctx->in_use -= (int64_t)b->tw_bytes; b->twmem = nullptr; b->tw_bytes = 0; }
-  if (b->items_d) { (void)hipFree(b->items_d); b->items_d = nullptr; }
+  if (b->work) { dev_release(b->ctx, b->work, b->work_bytes); b->ctx->in_use -= (int64_t)b->work_bytes; b->work = nullptr; b->work_bytes = 0; }
+  if (b->twmem) { dev_release(b->ctx, b->twmem, b->tw_bytes); b->ctx->in_use -= (int64_t)b->tw_bytes; b->twmem = nullptr; b->tw_bytes = 0; }
+  if (b->items_d) { dev_release(b->ctx, b->items_d, b->items_bytes); b->items_d = nullptr; }
 }
 
 // Allocate workspace, twiddle tables and work-item lists for every request.
@@ -415,18 +481,21 @@ int build_layout(radis_ldm_batch* b) {
   // --- workspace
   size_t total = 0;
   size_t fill_lds = 0;
+  b->small_b = true;
+  for (const Req& R : b->reqs)
+    if (R.plan.n2 > LDM_CAP_B) b->small_b = false;
   for (Req& R : b->reqs) {
     R.tile_w = choose_tile(R.m_cap);
-    if ((size_t)R.m_cap * R.tile_w * 4 > (size_t)FILL_LDS_TARGET)
+    if ((size_t)R.m_cap * (R.tile_w + 1) * 4 > (size_t)FILL_LDS_TARGET)
       return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit the fill tile (max %d rows)", R.m_cap,
                   FILL_LDS_TARGET / (4 * MIN_TILE_W));
     const int64_t V = R.nv + 1;
     R.n_tiles = (int)((V + R.tile_w - 1) / R.tile_w);
     R.stride = (int64_t)align_up((size_t)V, 64);
-    fill_lds = std::max(fill_lds, (size_t)R.m_cap * R.tile_w * 4);
+    fill_lds = std::max(fill_lds, (size_t)R.m_cap * (R.tile_w + 1) * 4);
     const int n1 = R.plan.n1, n2 = R.plan.n2;
     R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
-    R.bB = std::min(floor_log2(LDM_CAP / n2), ctz(n1));
+    R.bB = std::min(floor_log2((b->small_b ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
     const int pcap = (R.m_cap + 1) / 2;
     R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
     R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
@@ -450,16 +519,12 @@ int build_layout(radis_ldm_batch* b) {
   if (b->ctx->budget > 0 && b->ctx->in_use + (int64_t)(total + twsz) > b->ctx->budget)
     return fail(RADIS_LDM_ENOMEM, "workspace of %zu bytes exceeds the device budget (%lld in use of %lld)",
                 total + twsz, (long long)b->ctx->in_use, (long long)b->ctx->budget);
-  if (hipMalloc(&b->work, std::max<size_t>(total, 256)) != hipSuccess) {
-    b->work = nullptr;
-    return fail(RADIS_LDM_ENOMEM, "hipMalloc of %zu workspace bytes failed", total);
-  }
+  b->work = dev_alloc(b->ctx, std::max<size_t>(total, 256));
+  if (!b->work) return fail(RADIS_LDM_ENOMEM, "device allocation of %zu workspace bytes failed", total);
   b->work_bytes = std::max<size_t>(total, 256);
   b->ctx->in_use += (int64_t)b->work_bytes;
-  if (hipMalloc(&b->twmem, twsz) != hipSuccess) {
-    b->twmem = nullptr;
-    return fail(RADIS_LDM_ENOMEM, "hipMalloc of %zu twiddle bytes failed", twsz);
-  }
+  b->twmem = dev_alloc(b->ctx, twsz);
+  if (!b->twmem) return fail(RADIS_LDM_ENOMEM, "device allocation of %zu twiddle bytes failed", twsz);
   b->tw_bytes = twsz;
   b->ctx->in_use += (int64_t)twsz;
   {
@@ -517,7 +582,9 @@ int build_layout(radis_ldm_batch* b) {
   std::vector<int4> all;
   for (auto* v : {&ax, &fi, &fa, &fb, &fc}) all.insert(all.end(), v->begin(), v->end());
   if (all.empty()) all.push_back(make_int4(0, 0, 0, 0));
-  HIPCHK(hipMalloc(&b->items_d, all.size() * sizeof(int4)));
+  b->items_bytes = all.size() * sizeof(int4);
+  b->items_d = (int4*)dev_alloc(b->ctx, b->items_bytes);
+  if (!b->items_d) return fail(RADIS_LDM_ENOMEM, "device allocation of work items failed");
   HIPCHK(hipMemcpy(b->items_d, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
   b->fill_lds = std::max<size_t>(fill_lds, 4);
   return 0;
@@ -576,15 +643,21 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
   items += b->n_fill;
   if (b->n_a)
     hipLaunchKernelGGL(k_fft_a, dim3(b->n_a), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d,
-                       (const LdmAxes*)b->axes_d, items);
+                       (const LdmAxes*)b->axes_d, items, b->n_a);
   if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][4], s));
   items += b->n_a;
-  if (b->n_b)
-    hipLaunchKernelGGL(k_fft_b, dim3(b->n_b), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d,
-                       (const LdmAxes*)b->axes_d, items);
+  if (b->n_b) {
+    if (b->small_b)
+      hipLaunchKernelGGL((k_fft_b<LDM_CAP_B, LDM_FFT_T_B>), dim3(b->n_b), dim3(LDM_FFT_T_B), 0, s,
+                         (const LdmJob*)b->jobs_d, (const LdmAxes*)b->axes_d, items);
+    else
+      hipLaunchKernelGGL((k_fft_b<LDM_CAP, LDM_FFT_T>), dim3(b->n_b), dim3(LDM_FFT_T), 0, s,
+                         (const LdmJob*)b->jobs_d, (const LdmAxes*)b->axes_d, items);
+  }
   if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][5], s));
   items += b->n_b;
-  if (b->n_c) hipLaunchKernelGGL(k_fft_c, dim3(b->n_c), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d, items);
+  if (b->n_c)
+    hipLaunchKernelGGL(k_fft_c, dim3(b->n_c), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d, items, b->n_c);
   if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][6], s));
   HIPCHK(hipGetLastError());
   return 0;
@@ -626,6 +699,11 @@ void refresh_jobs(radis_ldm_batch* b, int slot, const std::vector<float*>& outs)
 }
 
 }  // namespace
+
+radis_ldm_ctx::~radis_ldm_ctx() {
+  for (auto& kv : dev_cache) (void)hipFree(kv.second);
+  for (auto& kv : host_cache) (void)hipHostFree(kv.second);
+}
 
 // ============================================================================
 // C ABI
@@ -781,6 +859,38 @@ int radis_ldm_db_size(const radis_ldm_db* db, int64_t* n_lines) {
   return 0;
 }
 
+int radis_ldm_request_bytes(const radis_ldm_db* db, const radis_ldm_request* req, int64_t* bytes,
+                            radis_ldm_request_info* info) {
+  if (!db || !req || !bytes) return fail(RADIS_LDM_EINVAL, "null argument");
+  Req R;
+  int rc = plan_one(*db, *req, R);
+  if (rc) return rc;
+  R.m_cap = R.m_need;
+  R.tile_w = choose_tile(R.m_cap);
+  const int64_t V = R.nv + 1;
+  const int64_t stride = (int64_t)align_up((size_t)V, 64);
+  const int64_t tiles = (V + R.tile_w - 1) / R.tile_w;
+  const int64_t pcap = (R.m_cap + 1) / 2;
+  int64_t b = (int64_t)R.m_cap * stride * 4 + pcap * R.plan.L * 8 + R.plan.L * 8 + tiles * 16 + (int64_t)R.m_cap * 16;
+  // twiddle tables, plus alignment slack: an upper bound of what a batch of
+  // this one request allocates
+  b += (int64_t)(R.plan.n1 + R.plan.n2 + 1024 + (R.plan.L >> LDM_TW_LO_BITS) + 1) * 8 + 64 * 1024;
+  *bytes = b;
+  if (info) {
+    std::memset(info, 0, sizeof *info);
+    info->n_v = R.nv;
+    info->fft_len = R.plan.L;
+    info->n1 = R.plan.n1;
+    info->n2 = R.plan.n2;
+    info->line_begin = R.i0;
+    info->line_end = R.i1;
+    info->m_cap = R.m_cap;
+    info->tile_w = R.tile_w;
+    info->scale_log2 = R.scale_log2;
+  }
+  return 0;
+}
+
 int radis_ldm_batch_create(radis_ldm_ctx* ctx, const radis_ldm_db* db, int n, const radis_ldm_request* reqs,
                            radis_ldm_batch** out) {
   if (!ctx || !db || !reqs || !out || n <= 0) return fail(RADIS_LDM_EINVAL, "bad batch arguments");
@@ -801,12 +911,14 @@ int radis_ldm_batch_create(radis_ldm_ctx* ctx, const radis_ldm_db* db, int n, co
   }
   if (!rc) rc = build_layout(b);
   if (!rc) {
-    bool ok = hipMalloc(&b->jobs_d, sizeof(LdmJob) * n) == hipSuccess &&
-              hipMalloc(&b->axes_d, sizeof(LdmAxes) * n) == hipSuccess &&
-              hipMalloc(&b->res_d, sizeof(LdmResult) * n) == hipSuccess;
+    b->jobs_d = (LdmJob*)dev_alloc(ctx, sizeof(LdmJob) * n);
+    b->axes_d = (LdmAxes*)dev_alloc(ctx, sizeof(LdmAxes) * n);
+    b->res_d = (LdmResult*)dev_alloc(ctx, sizeof(LdmResult) * n);
+    bool ok = b->jobs_d && b->axes_d && b->res_d;
     for (int k = 0; k < 2 && ok; ++k) {
-      ok = hipHostMalloc((void**)&b->jobs_h[k], sizeof(LdmJob) * n) == hipSuccess &&
-           hipHostMalloc((void**)&b->res_h[k], sizeof(LdmResult) * n) == hipSuccess &&
+      b->jobs_h[k] = (LdmJob*)host_alloc(ctx, sizeof(LdmJob) * n);
+      b->res_h[k] = (LdmResult*)host_alloc(ctx, sizeof(LdmResult) * n);
+      ok = b->jobs_h[k] && b->res_h[k] &&
            hipEventCreateWithFlags(&b->ev_done[k], hipEventDisableTiming) == hipSuccess;
       if (ok) std::memset(b->res_h[k], 0, sizeof(LdmResult) * n);
       for (auto& e : b->st_ev[k])
@@ -1000,15 +1112,15 @@ void radis_ldm_batch_destroy(radis_ldm_batch* b) {
     if (b->pending[k] && b->ev_done[k]) (void)hipEventSynchronize(b->ev_done[k]);
   free_layout(b);
   for (int k = 0; k < 2; ++k) {
-    if (b->jobs_h[k]) (void)hipHostFree(b->jobs_h[k]);
-    if (b->res_h[k]) (void)hipHostFree(b->res_h[k]);
+    host_release(b->ctx, b->jobs_h[k], sizeof(LdmJob) * b->n);
+    host_release(b->ctx, b->res_h[k], sizeof(LdmResult) * b->n);
     if (b->ev_done[k]) (void)hipEventDestroy(b->ev_done[k]);
     for (auto& e : b->st_ev[k])
       if (e) (void)hipEventDestroy(e);
   }
-  if (b->jobs_d) (void)hipFree(b->jobs_d);
-  if (b->axes_d) (void)hipFree(b->axes_d);
-  if (b->res_d) (void)hipFree(b->res_d);
+  dev_release(b->ctx, b->jobs_d, sizeof(LdmJob) * b->n);
+  dev_release(b->ctx, b->axes_d, sizeof(LdmAxes) * b->n);
+  dev_release(b->ctx, b->res_d, sizeof(LdmResult) * b->n);
   if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
   radis_ldm_ctx* ctx = b->ctx;
   radis_ldm_db* db = b->db;

@@ -1,10 +1,10 @@
 // LDS-staged mixed-radix Stockham FFT for CDNA4 (gfx950).
 //
-// A workgroup of LDM_FFT_T threads transforms B independent columns of
-// length N held in LDS as buf[i*B + b] (element i of column b).  The column
-// index is the lane-fastest dimension, so every LDS access of a stage is a
-// contiguous run of B complex values whatever the Stockham stride is.
-// Radices 2, 3, 4, 5, 7, 8, 16 are supported; N*B <= LDM_CAP.
+// A workgroup of T threads transforms B independent columns of length N
+// held in LDS as buf[pad_idx(i*B + b)] (element i of column b), N*B <= CAP.
+// The column index is the lane-fastest dimension and the LDS index is
+// padded, so the strided Stockham accesses spread over the LDS banks.
+// Radices 2, 3, 4, 5, 7, 8, 16 are supported.
 //
 // Stage s (radix R, Ns = product of the earlier radices) of the autosort
 // Stockham DIT recursion:
@@ -172,10 +172,18 @@ __device__ __forceinline__ void dft(float2* v) {
   else dft_odd<R, FWD>(v);
 }
 
-template <int R, bool FWD>
+// LDS slot of element e: two complex of padding after every 32 break the
+// power-of-two strides of the Stockham stages across the LDS banks.
+__device__ __forceinline__ int pad_idx(int e) { return e + ((e >> 5) << 1); }
+template <int CAP>
+struct LdsSize {
+  static constexpr int value = CAP + (CAP >> 4);
+};
+
+template <int R, bool FWD, int CAP, int T>
 __device__ __noinline__ void stockham_stage(float2* __restrict__ buf, int N, int bshift, int Ns,
-                                               const float2* __restrict__ tw) {
-  constexpr int MAXIT = (LDM_CAP + R * LDM_FFT_T - 1) / (R * LDM_FFT_T);
+                                            const float2* __restrict__ tw) {
+  constexpr int MAXIT = (CAP + R * T - 1) / (R * T);
   const int tid = threadIdx.x;
   const int NR = N / R;
   const int bmask = (1 << bshift) - 1;
@@ -184,12 +192,12 @@ __device__ __noinline__ void stockham_stage(float2* __restrict__ buf, int N, int
   float2 v[MAXIT][R];
 #pragma unroll
   for (int s = 0; s < MAXIT; ++s) {
-    const int it = tid + s * LDM_FFT_T;
+    const int it = tid + s * T;
     if (it < nb) {
       const int j = it >> bshift;
       const int b = it & bmask;
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[s][q] = buf[((j + q * NR) << bshift) + b];
+      for (int q = 0; q < R; ++q) v[s][q] = buf[pad_idx(((j + q * NR) << bshift) + b)];
       if (Ns > 1) {
         // W_{Ns*R}^(q*k) as powers of one table entry: w^2m = (w^m)^2,
         // w^(2m+1) = w^2m * w (at most six rounding steps for R = 16)
@@ -207,34 +215,35 @@ __device__ __noinline__ void stockham_stage(float2* __restrict__ buf, int N, int
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < MAXIT; ++s) {
-    const int it = tid + s * LDM_FFT_T;
+    const int it = tid + s * T;
     if (it < nb) {
       const int j = it >> bshift;
       const int b = it & bmask;
       const int k = j % Ns;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int q = 0; q < R; ++q) buf[((base + q * Ns) << bshift) + b] = v[s][q];
+      for (int q = 0; q < R; ++q) buf[pad_idx(((base + q * Ns) << bshift) + b)] = v[s][q];
     }
   }
   __syncthreads();
 }
 
-// Full transform of the B = 2^bshift columns in buf (forward: exp(-2 pi i nk/N)).
-template <bool FWD>
+// Full transform of the B = 2^bshift columns in buf (forward: exp(-2 pi i nk/N)),
+// N*B <= CAP, executed by T threads.
+template <bool FWD, int CAP, int T>
 __device__ void lds_fft(float2* __restrict__ buf, int N, int bshift, const int* rad, int nrad,
                         const float2* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s < nrad; ++s) {
     const int R = rad[s];
     switch (R) {
-      case 16: stockham_stage<16, FWD>(buf, N, bshift, Ns, tw); break;
-      case 8: stockham_stage<8, FWD>(buf, N, bshift, Ns, tw); break;
-      case 4: stockham_stage<4, FWD>(buf, N, bshift, Ns, tw); break;
-      case 2: stockham_stage<2, FWD>(buf, N, bshift, Ns, tw); break;
-      case 3: stockham_stage<3, FWD>(buf, N, bshift, Ns, tw); break;
-      case 5: stockham_stage<5, FWD>(buf, N, bshift, Ns, tw); break;
-      case 7: stockham_stage<7, FWD>(buf, N, bshift, Ns, tw); break;
+      case 16: stockham_stage<16, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
+      case 8: stockham_stage<8, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
+      case 4: stockham_stage<4, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
+      case 2: stockham_stage<2, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
+      case 3: stockham_stage<3, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
+      case 5: stockham_stage<5, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
+      case 7: stockham_stage<7, FWD, CAP, T>(buf, N, bshift, Ns, tw); break;
       default: break;
     }
     Ns *= R;

@@ -13,10 +13,12 @@
 #define LDM_MAX_AX 64      // maximum N_G or N_L
 #define LDM_MAX_ISO 16     // isotopologue slots
 #define LDM_MAX_RAD 12     // radix stages per sub-FFT
-#define LDM_CAP 8192       // complex elements held in LDS by one FFT workgroup
-#define LDM_FFT_T 512      // threads of an FFT workgroup
+#define LDM_CAP 8192       // complex elements held in LDS by a pass A/C workgroup (and a large pass B one)
+#define LDM_FFT_T 512      // threads of a pass A/C workgroup
+#define LDM_CAP_B 2048     // complex elements held by a small pass B workgroup
+#define LDM_FFT_T_B 256    // threads of a small pass B workgroup
 #define LDM_FILL_T 256     // threads of a fill workgroup
-#define LDM_AX_CHUNK 8192  // lines per axis-reduction workgroup
+#define LDM_AX_CHUNK 2048  // lines per axis-reduction workgroup
 #define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
 #define LDM_NFOLD_MAX 100000
 
@@ -53,6 +55,12 @@ struct LdmJob {
   double dxG, dxL, fold_thresh;
   double shift_margin;           // max|delta_air|*p_atm + 2*delta_w
   float c_stim_T, c_stim_R;      // -hc_k/T, -hc_k/Tref (stimulated emission, FP32)
+  float log_r_f;                 // log(Tref/T), FP32
+  float pa_air_f, pa_self_f;     // p_atm*(1-x), p_atm*x, FP32
+  float lgc_f[LDM_MAX_ISO];      // log(2*dopp_iso/c_CGS): log FWHM_G = log(nu0) + lgc
+  float laG0_f, inv_dxG_f;       // first G-axis node (log) and 1/dxG, FP32 (set by k_finalize)
+  float laL0_f, inv_dxL_f;       // same for the L axis (set by k_finalize)
+  int fp64_widths;               // 1: log-width positions always in FP64
   float out_scale;               // 1/(L*wstep)
   float inv_Lws;                 // 1/(L*wstep) (frequency step of the FT grid)
   float half_inv_ws;             // 1/(2*wstep)

@@ -82,6 +82,13 @@ __device__ __forceinline__ double fwhm_lorentz(const LdmJob& J, double g_air, do
   return (air + slf) * 2.0;
 }
 
+// FP32 Lorentzian FWHM (relative error ~1e-6), for filtering and fast paths.
+__device__ __forceinline__ float fwhm_lorentz_f32(const LdmJob& J, float ga, float gs, float na, float ns) {
+  const float ra = __expf(na * J.log_r_f);
+  const float rs = (ns == na) ? ra : __expf(ns * J.log_r_f);
+  return 2.0f * (ra * ga * J.pa_air_f + rs * gs * J.pa_self_f);
+}
+
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long double_to_bits(double d) {
   return (unsigned long long)__double_as_longlong(d);
@@ -91,7 +98,7 @@ __device__ __forceinline__ unsigned long long double_to_bits(double d) {
 // k_axis: min/max of the Lorentzian (and, for several isotopologues, the
 // Gaussian) FWHM over each request's window; radis/lbl/broadening.py:1823-1852.
 // Positive doubles order like their bit patterns, so the reduction uses
-// unsigned 64-bit atomics.  Item = (job, first line, last line).
+// unsigned 64-bit atomics.  Item = (job, chunk of LDM_AX_CHUNK lines).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                               LdmDb db) {
@@ -101,13 +108,20 @@ __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const i
   long long e = a + LDM_AX_CHUNK;
   if (e > J.i1) e = J.i1;
   unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
+  // FP32 filter: a line can only be this thread's exact (FP64) extreme if its
+  // FP32 width is within the FP32 error bound of the running FP32 extreme.
+  float fmin = 3.0e38f, fmax = 0.0f;
   for (long long i = a + threadIdx.x; i < e; i += blockDim.x) {
-    const double na = db.n_air[i];
-    const double ns = db.n_self ? (double)db.n_self[i] : na;
-    const double wl = fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, ns);
-    const unsigned long long bl = double_to_bits(wl);
-    lmin = bl < lmin ? bl : lmin;
-    lmax = bl > lmax ? bl : lmax;
+    const float ga = db.g_air[i], gs = db.g_self[i], na = db.n_air[i];
+    const float ns = db.n_self ? db.n_self[i] : na;
+    const float wf = fwhm_lorentz_f32(J, ga, gs, na, ns);
+    const bool cmin = wf <= fmin * (1.0f + 1e-5f);
+    const bool cmax = wf >= fmax * (1.0f - 1e-5f);
+    if (cmin || cmax) {
+      const unsigned long long bl = double_to_bits(fwhm_lorentz(J, ga, gs, na, ns));
+      if (cmin) { lmin = bl < lmin ? bl : lmin; fmin = wf < fmin ? wf : fmin; }
+      if (cmax) { lmax = bl > lmax ? bl : lmax; fmax = wf > fmax ? wf : fmax; }
+    }
     if (!J.single_iso) {
       const int iso = db.iso ? db.iso[i] : 1;
       const unsigned long long bg = double_to_bits(fwhm_gauss(db.nu0[i], J.dopp[iso]));
@@ -197,6 +211,10 @@ __global__ __launch_bounds__(64) void k_finalize(LdmJob* __restrict__ jobs, LdmA
           }
           for (int i = 0; i + 1 < ng; ++i) A.slG[i] = 1.0 / (A.laG[i + 1] - A.laG[i]);
           for (int i = 0; i + 1 < nl; ++i) A.slL[i] = 1.0 / (A.laL[i + 1] - A.laL[i]);
+          J.laG0_f = (float)A.laG[0];
+          J.laL0_f = (float)A.laL[0];
+          J.inv_dxG_f = (float)(1.0 / J.dxG);
+          J.inv_dxL_f = (float)(1.0 / J.dxL);
           if (ng * nl > J.m_cap) status |= LDM_ST_CAP;
         }
       }
@@ -275,18 +293,31 @@ __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, 
   J.tiles[2 * t + 1] = hi;
 }
 
-// Everything the fill needs from one line, in FP64 where NumPy parity of
-// the integer indices requires it:
-//   shifted centre        radis/lbl/base.py:2351-2360
+// Everything the fill needs from one line:
+//   shifted centre        radis/lbl/base.py:2351-2360                 (FP64)
 //   np.interp indices     radis/lbl/broadening.py:2151-2154, 2253-2263
 //   FWHM_G, FWHM_L        radis/lbl/broadening.py:170-172, 341-383
 //   S(T)                  radis/lbl/base.py:2664-2674 (Boltzmann and Q ratio FP64)
 // I = S(T) * number density * 2^-scale in FP32.
+//
+// The grid index and its fraction are always FP64 (np.interp rule).  The two
+// log-width positions are first evaluated in FP32 (relative error of the
+// widths below 2e-6, i.e. < 2e-5 in axis units); only when a position lies
+// within LDM_TIE_EPS of an axis node, or J.fp64_widths is set, are they
+// recomputed in FP64 with NumPy's rule, so the integer indices always equal
+// the FP64 ones.
+#define LDM_TIE_EPS 2e-4f
+
 struct LineTerms {
   long long k0;
   int l0, m0;
   float tv, tg, tl, I;
 };
+
+__device__ __forceinline__ bool near_node(float p, int n) {
+  const float f = p - floorf(p);
+  return f < LDM_TIE_EPS || f > 1.0f - LDM_TIE_EPS || p < LDM_TIE_EPS || p > (float)(n - 1) - LDM_TIE_EPS;
+}
 
 __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, long long i, const double* laG,
                                            const double* slG, const double* laL, const double* slL, int ng, int nl,
@@ -301,35 +332,52 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
   t.k0 = (long long)pos;
   t.tv = (float)(pos - (double)t.k0);
   const int iso = db.iso ? db.iso[i] : 1;
-  const double na = db.n_air[i];
-  const double ns = db.n_self ? (double)db.n_self[i] : na;
-  const double wl = fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, ns);
-  const double wg = fwhm_gauss(nu0, J.dopp[iso]);
-  const double pg = axis_position(log(wg), laG, slG, ng);
-  const double pl = axis_position(log(wl), laL, slL, nl);
-  t.l0 = (int)pg;
-  t.m0 = (int)pl;
-  t.tg = (float)(pg - (double)t.l0);
-  t.tl = (float)(pl - (double)t.m0);
+  const float ga = db.g_air[i], gs = db.g_self[i], na = db.n_air[i];
+  const float ns = db.n_self ? db.n_self[i] : na;
+  const float nuf = (float)nu0;
+  // FP32 positions on the log axes
+  float pg = (__logf(nuf) + J.lgc_f[iso] - J.laG0_f) * J.inv_dxG_f;
+  float pl = (__logf(fwhm_lorentz_f32(J, ga, gs, na, ns)) - J.laL0_f) * J.inv_dxL_f;
+  const bool exact_g = J.fp64_widths || near_node(pg, ng);
+  const bool exact_l = J.fp64_widths || near_node(pl, nl);
+  if (exact_g) {
+    const double p = axis_position(log(fwhm_gauss(nu0, J.dopp[iso])), laG, slG, ng);
+    t.l0 = (int)p;
+    t.tg = (float)(p - (double)t.l0);
+  } else {
+    const float f = floorf(pg);
+    t.l0 = (int)f;
+    t.tg = pg - f;
+  }
+  if (exact_l) {
+    const double p = axis_position(log(fwhm_lorentz(J, ga, gs, na, ns)), laL, slL, nl);
+    t.m0 = (int)p;
+    t.tl = (float)(p - (double)t.m0);
+  } else {
+    const float f = floorf(pl);
+    t.m0 = (int)f;
+    t.tl = pl - f;
+  }
   if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
   if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
   const double s64 = (double)db.S_ref[i] * J.qratio[iso] * exp(J.c_boltz * (double)db.El[i]) * J.i_scale;
-  const float nuf = (float)nu0;
   const float stim = expm1f(J.c_stim_T * nuf) / expm1f(J.c_stim_R * nuf);
   t.I = (float)s64 * stim;
 }
 
 // ---------------------------------------------------------------------------
 // k_fill: one workgroup per (request, tile of W grid points).  Every line
-// whose 2x2x2 stencil touches the tile computes, in FP64:
-//   shifted centre        radis/lbl/base.py:2351-2360
-//   S(T) (Boltzmann, Q)   radis/lbl/base.py:2664-2674
-//   FWHM_G, FWHM_L        radis/lbl/broadening.py:170-172, 341-383
-//   np.interp indices     radis/lbl/broadening.py:2151-2154, 2253-2263
-// then adds the simple weights (2305-2318) of its corners that fall in the
-// tile into LDS with ds_add_f32.  The tile is written to HBM once, so the
-// LDM needs no clearing and no global atomics.  Row index r = l*N_L + m.
+// whose 2x2x2 stencil touches the tile gets its LineTerms, then adds the
+// simple weights (radis/lbl/broadening.py:2305-2318) of its corners that
+// fall in the tile into LDS with ds_add_f32.  The tile is written to HBM
+// once, so the LDM needs no clearing and no global atomics.  Row index
+// r = l*N_L + m; LDS row stride W+1 so rows fall on different banks.
+// The tile's lines are split into 16 contiguous chunks, one per group of 16
+// lanes: a wave works on 4 distant parts of the tile at once, which keeps
+// its 64 atomics of one corner off the same LDS addresses.
 // ---------------------------------------------------------------------------
+#define LDM_FILL_GROUP 16
+
 __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
                                                      const int4* __restrict__ items, LdmDb db) {
@@ -341,18 +389,23 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   const int M = A.m;
   if (M == 0 || J.status) return;
   const int ng = A.ng, nl = A.nl;
-  const int W = J.tile_w;
+  const int W = J.tile_w, Wp = W + 1;
   const long long k_lo = (long long)it.y * W;
   long long k_hi = k_lo + W;
   if (k_hi > J.v) k_hi = J.v;
-  for (int e = threadIdx.x; e < M * W; e += blockDim.x) acc[e] = 0.0f;
+  for (int e = threadIdx.x; e < M * Wp; e += blockDim.x) acc[e] = 0.0f;
   for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
     s_laG[e] = A.laG[e]; s_slG[e] = A.slG[e];
     s_laL[e] = A.laL[e]; s_slL[e] = A.slL[e];
   }
   __syncthreads();
   const long long lo = J.tiles[2 * it.y], hi = J.tiles[2 * it.y + 1];
-  for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+  const int ngroups = LDM_FILL_T / LDM_FILL_GROUP;
+  const int gid = threadIdx.x / LDM_FILL_GROUP, lane = threadIdx.x % LDM_FILL_GROUP;
+  const long long per = (hi - lo + ngroups - 1) / ngroups;
+  const long long g0 = lo + gid * per;
+  const long long g1 = (g0 + per < hi) ? g0 + per : hi;
+  for (long long i = g0 + lane; i < g1; i += LDM_FILL_GROUP) {
     LineTerms t;
     line_terms(J, db, i, s_laG, s_slG, s_laL, s_slL, ng, nl, t);
     const long long k0 = t.k0;
@@ -371,7 +424,7 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
           const int r = (l0 + b) * nl + (m0 + d);
-          atomicAdd(&acc[r * W + c], iv[a] * ag[b] * al[d]);
+          atomicAdd(&acc[r * Wp + c], iv[a] * ag[b] * al[d]);
         }
     }
   }
@@ -379,7 +432,7 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   const int wlen = (int)(k_hi - k_lo);
   for (int e = threadIdx.x; e < M * W; e += blockDim.x) {
     const int r = e / W, c = e - (e / W) * W;
-    if (c < wlen) J.ldm[(long long)r * J.ldm_stride + k_lo + c] = acc[e];
+    if (c < wlen) J.ldm[(long long)r * J.ldm_stride + k_lo + c] = acc[r * Wp + c];
   }
 }
 
@@ -390,6 +443,16 @@ __device__ __forceinline__ float2 twiddle_L(const LdmJob& J, long long e) {
   return cmul(h, l);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Map hardware block h to a logical block so that consecutive logical
+// blocks (neighbouring column blocks, which share cache lines) run on the
+// same XCD at about the same time.
+__device__ __forceinline__ int xcd_block(int h, int G) {
+  const int per = G >> 3, rem = G & 7;
+  const int x = h & 7, s = h >> 3;
+  return x * per + (x < rem ? x : rem) + s;
+}
+
 // ---------------------------------------------------------------------------
 // k_fft_a: item = (job, pair p, column block cb).  Packs LDM rows 2p and
 // 2p+1 as z = x_2p + i x_2p+1 (rows are zero beyond V), runs the N1-point
@@ -398,9 +461,9 @@ __device__ __forceinline__ float2 twiddle_L(const LdmJob& J, long long e) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(LDM_FFT_T) void k_fft_a(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
-                                                     const int4* __restrict__ items) {
-  __shared__ float2 buf[LDM_CAP];
-  const int4 it = items[blockIdx.x];
+                                                     const int4* __restrict__ items, int n_items) {
+  __shared__ float2 buf[LdsSize<LDM_CAP>::value];
+  const int4 it = items[xcd_block(blockIdx.x, n_items)];
   const LdmJob& J = jobs[it.x];
   const int p = it.y;
   if (J.status || p >= axes[it.x].p) return;
@@ -418,16 +481,15 @@ __global__ __launch_bounds__(LDM_FFT_T) void k_fft_a(const LdmJob* __restrict__ 
       xa = ra[n];
       if (has_b) xb = rb[n];
     }
-    buf[e] = make_float2(xa, xb);
+    buf[pad_idx(e)] = make_float2(xa, xb);
   }
   __syncthreads();
-  lds_fft<true>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
+  lds_fft<true, LDM_CAP, LDM_FFT_T>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
   float2* Y = J.ybuf + (long long)p * J.L;
   for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
     const int k1 = e >> bs, b = e & (B - 1);
     const long long n2 = n2_0 + b;
-    const long long ex = (n2 * k1) % J.L;
-    Y[(long long)k1 * N2 + n2] = cmul(buf[e], twiddle_L(J, ex));
+    Y[(long long)k1 * N2 + n2] = cmul(buf[pad_idx(e)], twiddle_L(J, n2 * k1));
   }
 }
 
@@ -455,11 +517,12 @@ __device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, long long 
 // and then B[k1][n2'] = W_L^(-n2' k1) * sum_k2 A[k1 + N1 k2] W_N2^(-n2' k2).
 // With real LDM rows x_a, x_b and real symmetric F, Re(IFFT(A)) equals
 // IFFT(X_a F_a + X_b F_b): the pair packing needs no real-FFT unpacking.
+// CAP/T select a small tile (many workgroups, N2 <= 2048) or a large one.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(LDM_FFT_T) void k_fft_b(const LdmJob* __restrict__ jobs,
-                                                     const LdmAxes* __restrict__ axes,
-                                                     const int4* __restrict__ items) {
-  __shared__ float2 buf[LDM_CAP];
+template <int CAP, int T>
+__global__ __launch_bounds__(T) void k_fft_b(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
+                                             const int4* __restrict__ items) {
+  __shared__ float2 buf[LdsSize<CAP>::value];
   const int4 it = items[blockIdx.x];
   const LdmJob& J = jobs[it.x];
   if (J.status) return;
@@ -468,30 +531,30 @@ __global__ __launch_bounds__(LDM_FFT_T) void k_fft_b(const LdmJob* __restrict__ 
   const long long L = J.L;
   const int P = axes[it.x].p, M = axes[it.x].m;
   const int total = N2 << bs;
-  constexpr int NACC = LDM_CAP / LDM_FFT_T;
+  constexpr int NACC = CAP / T;
   float2 acc[NACC];
 #pragma unroll
   for (int s = 0; s < NACC; ++s) acc[s] = make_float2(0.0f, 0.0f);
   for (int p = 0; p < P; ++p) {
     const float2* Y = J.ybuf + (long long)p * L;
     // coalesced: n2 fastest in global memory, transposed into [n2][b] in LDS
-    for (int e = threadIdx.x; e < total; e += LDM_FFT_T) {
+    for (int e = threadIdx.x; e < total; e += T) {
       const int b = e / N2, n2 = e - (e / N2) * N2;
-      buf[(n2 << bs) + b] = Y[(long long)(k1_0 + b) * N2 + n2];
+      buf[pad_idx((n2 << bs) + b)] = Y[(long long)(k1_0 + b) * N2 + n2];
     }
     __syncthreads();
-    lds_fft<true>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
+    lds_fft<true, CAP, T>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
     const int ra = 2 * p, rb = 2 * p + 1;
 #pragma unroll
     for (int s = 0; s < NACC; ++s) {
-      const int e = threadIdx.x + s * LDM_FFT_T;
+      const int e = threadIdx.x + s * T;
       if (e < total) {
         const int k2 = e >> bs, b = e & (B - 1);
         const long long k = (long long)(k1_0 + b) + (long long)N1 * k2;
         const long long kk = (k <= L - k) ? k : (L - k);
         const float fa = lineshape_ft(J, ra, kk);
         const float fb = (rb < M) ? lineshape_ft(J, rb, kk) : 0.0f;
-        const float2 z = buf[e];
+        const float2 z = buf[pad_idx(e)];
         acc[s].x += z.x * fa + z.y * fb;
         acc[s].y += z.y * fa - z.x * fb;
       }
@@ -500,16 +563,15 @@ __global__ __launch_bounds__(LDM_FFT_T) void k_fft_b(const LdmJob* __restrict__ 
   }
 #pragma unroll
   for (int s = 0; s < NACC; ++s) {
-    const int e = threadIdx.x + s * LDM_FFT_T;
-    if (e < total) buf[e] = acc[s];
+    const int e = threadIdx.x + s * T;
+    if (e < total) buf[pad_idx(e)] = acc[s];
   }
   __syncthreads();
-  lds_fft<false>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
-  for (int e = threadIdx.x; e < total; e += LDM_FFT_T) {
+  lds_fft<false, CAP, T>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
+  for (int e = threadIdx.x; e < total; e += T) {
     const int b = e / N2, n2 = e - (e / N2) * N2;
     const int k1 = k1_0 + b;
-    const long long ex = ((long long)n2 * k1) % L;
-    J.bbuf[(long long)k1 * N2 + n2] = cmulc(buf[(n2 << bs) + b], twiddle_L(J, ex));
+    J.bbuf[(long long)k1 * N2 + n2] = cmulc(buf[pad_idx((n2 << bs) + b)], twiddle_L(J, (long long)n2 * k1));
   }
 }
 
@@ -518,9 +580,10 @@ __global__ __launch_bounds__(LDM_FFT_T) void k_fft_b(const LdmJob* __restrict__ 
 // sum_k1 B[k1][n2] W_N1^(-n1 k1); abscoeff[n] = Re(y[n]) / (L wstep) for
 // n < N_v (radis/lbl/broadening.py:2529-2530 crop and /wstep).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(LDM_FFT_T) void k_fft_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items) {
-  __shared__ float2 buf[LDM_CAP];
-  const int4 it = items[blockIdx.x];
+__global__ __launch_bounds__(LDM_FFT_T) void k_fft_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items,
+                                                     int n_items) {
+  __shared__ float2 buf[LdsSize<LDM_CAP>::value];
+  const int4 it = items[xcd_block(blockIdx.x, n_items)];
   const LdmJob& J = jobs[it.x];
   const int N1 = J.n1, N2 = J.n2, bs = J.bA, B = 1 << bs;
   const long long n2_0 = (long long)it.z * B;
@@ -533,15 +596,15 @@ __global__ __launch_bounds__(LDM_FFT_T) void k_fft_c(const LdmJob* __restrict__ 
   }
   for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
     const int k1 = e >> bs, b = e & (B - 1);
-    buf[e] = J.bbuf[(long long)k1 * N2 + n2_0 + b];
+    buf[pad_idx(e)] = J.bbuf[(long long)k1 * N2 + n2_0 + b];
   }
   __syncthreads();
-  lds_fft<false>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
+  lds_fft<false, LDM_CAP, LDM_FFT_T>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
   const float sc = J.out_scale;
   for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
     const int n1 = e >> bs, b = e & (B - 1);
     const long long n = n2_0 + b + (long long)N2 * n1;
-    if (n < J.nv) J.out[n] = buf[e].x * sc;
+    if (n < J.nv) J.out[n] = buf[pad_idx(e)].x * sc;
   }
 }
 

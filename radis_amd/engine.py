@@ -150,6 +150,13 @@ class LineDatabase:
         """(i0, i1): lines with wmin <= nu0 <= wmax (radis/lbl/loader.py:2707)."""
         return int(np.searchsorted(self.nu0, wmin, "left")), int(np.searchsorted(self.nu0, wmax, "right"))
 
+    def request_bytes(self, req):
+        """(workspace bytes, planned RequestInfo) of one request, nothing allocated."""
+        b = c_int64()
+        inf = _native.RequestInfo()
+        check(self._lib.radis_ldm_request_bytes(self.handle, byref(req), byref(b), byref(inf)))
+        return int(b.value), inf
+
     def q_ratio(self, T, Tref):
         """Q(Tref)/Q(T) per iso slot; slots without lines get 1."""
         out = np.ones(MAX_ISO)
@@ -174,7 +181,7 @@ class LineDatabase:
 
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
-                 fft_len=0, q_ratio=None, n_v=0, line_window=None):
+                 fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
@@ -188,14 +195,16 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
         r.q_ratio[s] = float(q[s]) if s < q.size else 1.0
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
+    r.flags = _native.FP64_WIDTHS if fp64_widths else 0
     if line_window is not None:
         r.line_lo, r.line_hi = float(line_window[0]), float(line_window[1])
     return r
 
 
-def request_from_conditions(db, cond, fft_len=0):
+def request_from_conditions(db, cond, fft_len=0, fp64_widths=False):
     return make_request(db, cond.wmin, cond.wmax, cond.wstep, cond.Tgas, cond.pressure_bar,
-                        cond.mole_fraction, cond.dxG, cond.dxL, cond.Tref, fft_len=fft_len)
+                        cond.mole_fraction, cond.dxG, cond.dxL, cond.Tref, fft_len=fft_len,
+                        fp64_widths=fp64_widths)
 
 
 def plan_grid(wmin, wmax, wstep):

@@ -1,0 +1,244 @@
+"""Caller-side facade of the GPU path: ``SpectrumFactory.eq_spectrum_gpu``,
+``Spectrum.recalc_gpu`` / ``exit_gpu`` and ``calc_spectrum(mode='gpu')``
+(radis/lbl/factory.py:1027-1391, radis/spectrum/spectrum.py:6438-6577,
+radis/lbl/calc.py:37, 1014-1028).
+
+Only the parts of those callers that belong to the GPU path are reproduced:
+unit handling (K, bar, cm), the calculation grid, the synthesis itself, the
+derived quantities computed right after it (absorbance, emissivity,
+transmittance, Planck radiance; radis/lbl/factory.py:1284-1297) and the
+``NwL``/``NwG`` conditions.  Line lists are handed in directly (``lines=``);
+databank download and file parsing are not part of this package.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import engine
+from .engine import DXG_DEFAULT, DXL_DEFAULT, FOLDING_THRESH_DEFAULT
+
+# radis/phys/constants.py
+_H, _C, _KB = 6.62607015e-34, 2.99792458e8, 1.3806490e-23
+
+
+def planck_nm(lambda_nm, T):
+    """Blackbody radiance in mW/cm2/sr/nm at vacuum wavelength (nm);
+    radis/phys/blackbody.py:35-74."""
+    lbd = lambda_nm * 1e-9
+    with np.errstate(over="ignore"):
+        ipl = (2 * _H * _C**2 / lbd**5) * (1 / (np.exp(_H * _C / (lbd * _KB * T)) - 1))
+    return ipl * 1e-10
+
+
+def calc_radiance(wavenumber, emissivity, Tgas):
+    """radiance_noslit (mW/cm2/sr/nm) = emissivity * Planck(1e7/nu, T);
+    radis/spectrum/equations.py:21-44 with radis/phys/convert.py cm2nm."""
+    return emissivity * planck_nm(1e7 / wavenumber, Tgas)
+
+
+class Spectrum:
+    """Result of ``eq_spectrum_gpu`` holding the GPU application until
+    ``exit_gpu`` (radis/spectrum/spectrum.py:176-179, 366, 528)."""
+
+    UNITS = {"abscoeff": "cm-1", "absorbance": "", "emissivity_noslit": "", "transmittance_noslit": "",
+             "radiance_noslit": "mW/cm2/sr/nm", "wavenumber": "cm-1"}
+
+    def __init__(self, wavenumber, abscoeff, conditions, gpu_app=None, name=None):
+        self._w = wavenumber
+        self.conditions = dict(conditions)
+        self.gpu_app = gpu_app
+        self.name = name
+        self._q = {}
+        self._set_abscoeff(abscoeff)
+
+    def _set_abscoeff(self, abscoeff):
+        # radis/lbl/factory.py:1284-1297
+        L = self.conditions["path_length"]
+        absorbance = abscoeff * L
+        emissivity = -np.expm1(-absorbance)
+        self._q = {
+            "abscoeff": abscoeff,
+            "absorbance": absorbance,
+            "emissivity_noslit": emissivity,
+            "transmittance_noslit": 1 - emissivity,
+            "radiance_noslit": calc_radiance(self._w, emissivity, self.conditions["Tgas"]),
+        }
+
+    def get_vars(self):
+        return list(self._q)
+
+    def get(self, var, wunit="cm-1", Iunit="default", copy=True):
+        """(wavenumber, quantity); only cm-1 and the default units."""
+        if wunit not in ("cm-1", "default"):
+            raise NotImplementedError("only wunit='cm-1'")
+        if Iunit not in ("default", self.UNITS.get(var)):
+            raise NotImplementedError("only default units")
+        y = self._q[var]
+        return (self._w.copy(), y.copy()) if copy else (self._w, y)
+
+    def recalc_gpu(self, var="abscoeff", wunit="default", Iunit="default", pressure=None, Tgas=None,
+                   mole_fraction=None, path_length=None, slit_function=None):
+        """Recompute with new conditions on the resident GPU application;
+        radis/spectrum/spectrum.py:6438-6572."""
+        if self.gpu_app is None:
+            raise RuntimeError("GPU not initialized: recalc_gpu() needs a spectrum from eq_spectrum_gpu() "
+                               "that has not called exit_gpu()")
+        t0 = time.perf_counter()
+        for k, v in (("Tgas", Tgas), ("pressure", pressure), ("mole_fraction", mole_fraction),
+                     ("path_length", path_length), ("slit_function", slit_function)):
+            if v is not None:
+                self.conditions[k] = v
+        self.conditions["Tvib"] = self.conditions["Trot"] = self.conditions["Tgas"]
+        abscoeff, _ = self.gpu_app.iterate(self.conditions["pressure"], self.conditions["Tgas"],
+                                           self.conditions["mole_fraction"])
+        NwL, NwG = self.gpu_app.get_griddims()
+        self.conditions["NwL"], self.conditions["NwG"] = NwL, NwG
+        self._set_abscoeff(abscoeff[self.conditions["woutrange"][0]:self.conditions["woutrange"][1]]
+                           .astype(np.float64))
+        self.conditions["calculation_time"] = time.perf_counter() - t0
+        return self.get(var, wunit=wunit, Iunit=Iunit, copy=False)[1]
+
+    def exit_gpu(self, event=None):
+        """radis/spectrum/spectrum.py:6574-6577."""
+        if self.gpu_app is not None:
+            self.gpu_app.__del__()
+            self.gpu_app = None
+
+
+class SpectrumFactory:
+    """The parts of ``radis.lbl.factory.SpectrumFactory`` that drive the GPU
+    path (radis/lbl/factory.py:415-452 defaults)."""
+
+    def __init__(self, wavenum_min=None, wavenum_max=None, wstep=0.01, Tref=296, pressure=1.01325,
+                 mole_fraction=1, path_length=1, dxG=DXG_DEFAULT, dxL=DXL_DEFAULT,
+                 folding_thresh=FOLDING_THRESH_DEFAULT, lines=None, verbose=0, **kwargs):
+        if wavenum_min is None or wavenum_max is None:
+            raise ValueError("wavenum_min and wavenum_max (cm-1) are required")
+        if kwargs.get("neighbour_lines", 0):
+            raise NotImplementedError("neighbour_lines > 0")
+        self.wavenum_min, self.wavenum_max, self.wstep = float(wavenum_min), float(wavenum_max), float(wstep)
+        self.Tref = float(Tref)
+        self.pressure, self.mole_fraction, self.path_length = pressure, mole_fraction, path_length
+        self.dxG, self.dxL, self.folding_thresh = dxG, dxL, folding_thresh
+        self.verbose = verbose
+        self.lines = None
+        if lines is not None:
+            self.load_lines(lines)
+
+    def load_lines(self, lines):
+        """Use ``lines`` (a LineList or a mapping of columns) as the line
+        database; replaces load_databank/fetch_databank."""
+        self.lines = lines
+        return self
+
+    @property
+    def wavenumber(self):
+        # radis/lbl/factory.py:2590-2592
+        return np.arange(self.wavenum_min, self.wavenum_max + self.wstep, self.wstep)
+
+    def _line_columns(self):
+        L = self.lines
+        get = (lambda k: getattr(L, k, None)) if not isinstance(L, dict) else L.get
+        nu0 = np.asarray(get("nu0"), dtype=np.float64)
+        keep = (nu0 >= self.wavenum_min) & (nu0 <= self.wavenum_max)  # radis/lbl/loader.py:2707
+        col = lambda k: None if get(k) is None else np.asarray(get(k))[keep]
+        return dict(nu0=nu0[keep], S_ref=col("S_ref"), El=col("El"), g_air=col("g_air"), g_self=col("g_self"),
+                    n_air=col("n_air"), n_self=col("n_self"), delta=col("delta"), iso=col("iso"),
+                    molar_mass=get("molar_mass"), Q=get("Q"))
+
+    def eq_spectrum_gpu(self, Tgas, mole_fraction=None, diluent=None, path_length=None, pressure=None,
+                        name=None, backend="hip", device_id=0, exit_gpu=False, verbose=None):
+        """Equilibrium spectrum on the GPU; radis/lbl/factory.py:1027-1391."""
+        if backend not in ("hip", "vulkan"):
+            raise NotImplementedError(f"Unknown GPU backend {backend!r}")
+        if diluent not in (None, "air") and diluent != {"air": 1 - (mole_fraction or self.mole_fraction)}:
+            raise NotImplementedError("only air as diluent")
+        if self.lines is None:
+            raise ValueError("no line database: call load_lines() first")
+        if not np.isscalar(Tgas):
+            raise ValueError("Tgas should be float.")
+        if mole_fraction is not None:
+            self.mole_fraction = mole_fraction
+        if pressure is not None:
+            self.pressure = pressure
+        if path_length is not None:
+            self.path_length = path_length
+        t0 = time.perf_counter()
+        cols = self._line_columns()
+        w = self.wavenumber
+        app = _LineApp(self, cols, w.size, device_id)
+        abscoeff, _ = app.iterate(self.pressure, float(Tgas), self.mole_fraction)
+        NwL, NwG = app.get_griddims()
+        conditions = dict(
+            Tgas=float(Tgas), Tvib=float(Tgas), Trot=float(Tgas), pressure=self.pressure,
+            mole_fraction=self.mole_fraction, path_length=self.path_length, wstep=self.wstep,
+            wavenum_min=self.wavenum_min, wavenum_max=self.wavenum_max, dxG=self.dxG, dxL=self.dxL,
+            NwL=NwL, NwG=NwG, lines_calculated=int(cols["nu0"].size), thermal_equilibrium=True,
+            diluents={"air": 1 - self.mole_fraction}, gpu_backend="hip", spectral_points=int(w.size),
+            add_at_used="gpu-backend", woutrange=(0, int(w.size)), calculation_time=time.perf_counter() - t0,
+        )
+        s = Spectrum(w, abscoeff.astype(np.float64), conditions, gpu_app=None if exit_gpu else app, name=name)
+        if exit_gpu:
+            app.__del__()
+        return s
+
+    def eq_spectrum_gpu_interactive(self, *args, **kwargs):
+        raise NotImplementedError("matplotlib sliders are outside this package; use Spectrum.recalc_gpu")
+
+
+class _LineApp:
+    """gpuApp-like holder built directly from S(Tref) line columns (the
+    factory path does not go through the Einstein-A conversion)."""
+
+    def __init__(self, sf, cols, Nv, device_id):
+        self._dev = engine.Device(int(device_id))
+        self._db = engine.LineDatabase(self._dev, cols)
+        self.sf = sf
+        self.Nv = Nv
+        self._batch = None
+        self._outs = None
+        self._griddims = (0, 0)
+
+    def _req(self, p, T, x):
+        sf = self.sf
+        return engine.make_request(self._db, sf.wavenum_min, sf.wavenum_max, sf.wstep, T, p, x, sf.dxG, sf.dxL,
+                                   sf.Tref, sf.folding_thresh)
+
+    def iterate(self, p, T, x):
+        req = self._req(p, T, x)
+        if self._batch is None:
+            self._batch = engine.SpectrumBatch(self._db, [req])
+            self._outs = self._batch.alloc_outputs()
+        else:
+            self._batch.set_request(0, req)
+        self._batch.run(self._outs)
+        out = self._batch.results(self._outs)[0]
+        inf = self._batch.info(0)
+        self._griddims = (int(inf.n_l), int(inf.n_g))
+        return out, {}
+
+    def get_griddims(self):
+        return self._griddims
+
+    def __del__(self):
+        for name in ("_batch", "_db", "_dev"):
+            obj = getattr(self, name, None)
+            if obj is not None:
+                try:
+                    obj.close()
+                except Exception:
+                    pass
+                setattr(self, name, None)
+
+
+def calc_spectrum(wavenum_min, wavenum_max, Tgas, pressure=1.01325, mole_fraction=1, path_length=1,
+                  wstep=0.01, lines=None, mode="gpu", name=None, **kwargs):
+    """``calc_spectrum(..., mode='gpu')`` for one molecule whose lines are
+    given; radis/lbl/calc.py:37, 1014-1028."""
+    if mode != "gpu":
+        raise NotImplementedError("this package implements mode='gpu'")
+    sf = SpectrumFactory(wavenum_min, wavenum_max, wstep=wstep, pressure=pressure, mole_fraction=mole_fraction,
+                         path_length=path_length, lines=lines, **kwargs)
+    return sf.eq_spectrum_gpu(Tgas, name=name, exit_gpu=True)

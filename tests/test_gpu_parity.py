@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import ldm_oracle as oracle
-from radis_amd import engine, workloads
+from radis_amd import engine, hitran, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -37,15 +37,17 @@ def _oracle(lines, c, L, details=False):
                               c.dxG, c.dxL, Tref=c.Tref, L=L, return_details=details)
 
 
-def _check_indices(b, det, i=0):
+def _check_indices(b, det, i=0, frac_atol=3e-5):
+    """Integer indices bit-exact; grid fraction to FP32 rounding; log-axis
+    fractions to the FP32 fast-path bound (FP64 when requested)."""
     t = b.line_terms(i)
     assert t["k0"].size == det["ki0"].size
     np.testing.assert_array_equal(t["k0"], det["ki0"])
     np.testing.assert_array_equal(t["l0"], det["li0"])
     np.testing.assert_array_equal(t["m0"], det["mi0"])
     np.testing.assert_allclose(t["tv"], det["tvi"], atol=1e-6)
-    np.testing.assert_allclose(t["tg"], det["tGi"], atol=1e-6)
-    np.testing.assert_allclose(t["tl"], det["tLi"], atol=1e-6)
+    np.testing.assert_allclose(t["tg"], det["tGi"], atol=frac_atol)
+    np.testing.assert_allclose(t["tl"], det["tLi"], atol=frac_atol)
     return t
 
 
@@ -65,10 +67,15 @@ def test_co5k_full(dev):
     assert err <= REL_TOL
 
 
-def test_co5k_ldm_fill_matches_add_at(dev):
-    """The LDS-tiled fill equals np.add.at into the float64 LDM."""
+@pytest.mark.parametrize("fp64_widths,tol", [(True, 1e-6), (False, 6e-5)])
+def test_co5k_ldm_fill_matches_add_at(dev, fp64_widths, tol):
+    """The LDS-tiled fill equals np.add.at into the float64 LDM (to FP32
+    rounding with FP64 widths; to the fast path's fraction bound, 3e-5 of
+    an axis step per weight, otherwise)."""
     lines, c = workloads.workload("co_5k")
-    db, b, res = _run(dev, lines, [c])
+    db = engine.LineDatabase(dev, lines)
+    b = engine.SpectrumBatch(db, [engine.request_from_conditions(db, c, fp64_widths=fp64_widths)])
+    b.results(b.run())
     inf = b.info(0)
     ref, det = _oracle(lines, c, int(inf.fft_len), details=True)
     S = det["S"] * oracle.number_density(c.mole_fraction, c.pressure_bar, c.Tgas)
@@ -78,7 +85,7 @@ def test_co5k_ldm_fill_matches_add_at(dev):
     got = np.ldexp(b.ldm(0).astype(np.float64), inf.scale_log2)
     assert got.shape == want.shape
     scale = np.abs(want).max()
-    assert np.max(np.abs(got - want)) <= 1e-6 * scale
+    assert np.max(np.abs(got - want)) <= tol * scale
 
 
 def test_co2_subset_full_grid(dev):
@@ -139,3 +146,80 @@ def test_graph_replay_matches_eager(dev):
     got = b.results(outs)[0]
     ref = _oracle(lines, c2, int(b.info(0).fft_len))
     assert oracle.rel_peak_error(got, ref) <= REL_TOL
+
+
+def test_co5k_fp64_widths(dev):
+    """Full-FP64 width path: fractions to FP32 rounding of the FP64 values."""
+    lines, c = workloads.workload("co_5k")
+    db = engine.LineDatabase(dev, lines)
+    b = engine.SpectrumBatch(db, [engine.request_from_conditions(db, c, fp64_widths=True)])
+    res = b.results(b.run())
+    inf = b.info(0)
+    ref, det = _oracle(lines, c, int(inf.fft_len), details=True)
+    _check_indices(b, det, frac_atol=1e-6)
+    assert oracle.rel_peak_error(res[0], ref) <= REL_TOL
+
+
+def test_hitran_fixture_isotopes_and_shift(dev):
+    """Real HITRAN CO lines (3 isotopologues, pressure shifts), the
+    reference's test fixture test/files/hitran_co_3iso_2000_2300cm.par."""
+    lines = hitran.line_list("tests/golden/hitran_co_3iso_2000_2300cm.par")
+    conds = [workloads.Conditions(2000.0, 2300.0, 0.002, 1500.0, 2.0, 0.5),
+             workloads.Conditions(2100.0, 2200.0, 0.001, 300.0, 1.0, 0.95, dxG=0.2, dxL=0.2),
+             workloads.Conditions(2000.0, 2300.0, 0.01, 3000.0, 0.01, 0.1)]
+    db, b, res = _run(dev, lines, conds)
+    for i, c in enumerate(conds):
+        inf = b.info(i)
+        ref, det = _oracle(lines, c, int(inf.fft_len), details=True)
+        _check_indices(b, det, i)
+        err = oracle.rel_peak_error(res[i], ref)
+        print(f"hitran cond {i}: err {err:.3e} NG={inf.n_g} NL={inf.n_l}")
+        assert err <= REL_TOL
+
+
+def test_lines_outside_grid_are_clamped(dev):
+    """Lines beyond the grid ends land on the edge points (np.interp clamp)."""
+    lines = workloads.co_like(4000, seed=3)
+    c = workloads.Conditions(2000.0, 2050.0, 0.002, 800.0, 1.0, 0.3, dxG=0.2, dxL=0.2)
+    db = engine.LineDatabase(dev, lines)
+    req = engine.make_request(db, c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction,
+                              c.dxG, c.dxL, line_window=(1990.0, 2060.0))
+    b = engine.SpectrumBatch(db, [req])
+    got = b.results(b.run())[0]
+    inf = b.info(0)
+    sub = lines.subset(slice(*db.window(1990.0, 2060.0)))
+    w = oracle.wavenumber_grid(c.wmin, c.wmax, c.wstep)
+    S, nu, wG, wL = oracle.line_parameters(sub.as_dict(), c.Tgas, c.pressure_bar, c.mole_fraction)
+    wLa, wGa = oracle.init_w_axis(wL, c.dxL), oracle.init_w_axis(wG, c.dxG)
+    k0, k1, tv = oracle.get_indices(nu, w)
+    l0, l1, tg = oracle.get_indices(np.log(wG), np.log(wGa))
+    m0, m1, tl = oracle.get_indices(np.log(wL), np.log(wLa))
+    assert np.any(k0 == 0) and np.any(k0 == w.size - 1)
+    L = int(inf.fft_len)
+    LDM = oracle.ldm_scatter(L, wGa.size, wLa.size, k0, k1, tv, l0, l1, tg, m0, m1, tl, S)
+    ref = oracle.ldm_apply_fft(LDM, oracle.lineshape_templates(L, c.wstep, wGa, wLa), w.size, c.wstep)
+    ref = ref * oracle.number_density(c.mole_fraction, c.pressure_bar, c.Tgas)
+    assert oracle.rel_peak_error(got, ref) <= REL_TOL
+
+
+def test_empty_window_gives_zeros(dev):
+    lines = workloads.co_like(1000, seed=4)
+    db = engine.LineDatabase(dev, lines)
+    req = engine.make_request(db, 2400.0, 2410.0, 0.01, 1000.0, 1.0, 0.1, dxG=0.2, dxL=0.2)
+    b = engine.SpectrumBatch(db, [req])
+    got = b.results(b.run())[0]
+    assert got.shape == (np.arange(2400.0, 2410.0 + 0.01, 0.01).size,) and np.all(got == 0.0)
+    assert b.info(0).n_g == 0
+
+
+def test_extreme_intensities_use_power_of_two_scale(dev):
+    """E'' spread of the 100M-line H2O-like list at 2500 K makes S(T) exceed
+    the FP32 range; the library scales by 2**-s and reports s."""
+    lines = workloads.h2o_like(200_000, seed=8, chunk=200_000)
+    c = workloads.Conditions(2000.0, 2100.0, 0.005, 2500.0, 5.0 * workloads.ATM, 0.1)
+    db, b, res = _run(dev, lines, [c])
+    inf = b.info(0)
+    assert inf.scale_log2 != 0
+    ref = _oracle(lines, c, int(inf.fft_len))
+    assert not np.isfinite(np.float32(ref.max())) or ref.max() > 1e30
+    assert oracle.rel_peak_error(res[0], ref) <= REL_TOL

@@ -1,0 +1,211 @@
+"""Pins the CPU oracle (oracle/ldm_oracle.py) before anything is compared
+with it: the reference's known-answer test, NumPy's own routines, closed
+forms and brute force on tiny inputs.  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import voigt_profile
+
+from oracle import ldm_oracle as o
+from radis_amd import hitran, workloads
+
+
+# --- grid: lbl/factory.py:2590-2617 ---------------------------------------------
+@pytest.mark.parametrize("wmin,wmax,wstep,n", [(1900.0, 2300.0, 0.002, 200001), (2000.0, 2500.0, 0.001, 500002),
+                                               (1000.0, 4000.0, 0.005, 600001)])
+def test_grid_sizes_and_fill_rule(wmin, wmax, wstep, n):
+    w = o.wavenumber_grid(wmin, wmax, wstep)
+    assert w.size == n
+    assert w.size == math.ceil(((wmax + wstep) - wmin) / wstep)
+    delta = (wmin + wstep) - wmin
+    np.testing.assert_array_equal(w, wmin + np.arange(n, dtype=np.float64) * delta)
+
+
+# --- indices: np.interp semantics, lbl/broadening.py:2151-2154 -----------------
+def test_get_indices_matches_scalar_interp_rule():
+    rng = np.random.default_rng(0)
+    w = o.wavenumber_grid(2000.0, 2001.0, 0.001)
+    x = np.concatenate([rng.uniform(1999.9, 2001.1, 3000), w[::37], [w[0], w[-1], w[-1] + 1e-9, w[0] - 1e-9]])
+    k0, k1, t = o.get_indices(x, w)
+    for xi, ki, ti in zip(x, k0, t):
+        pos = o.interp_position_scalar(xi, w)
+        assert ki == int(pos) and ti == pos - int(pos)
+    assert np.all(k1 == k0 + 1)
+
+
+def test_naive_floor_is_not_interp():
+    """A plausible shortcut, floor((x-wmin)/wstep), disagrees with np.interp
+    on some lines: the index rule is not replaceable by it."""
+    rng = np.random.default_rng(1)
+    w = o.wavenumber_grid(1900.0, 2300.0, 0.002)
+    x = rng.uniform(1900.0, 2299.9, 2_000_000)
+    k0, _, _ = o.get_indices(x, w)
+    naive = np.floor((x - 1900.0) / 0.002).astype(np.int64)
+    assert np.any(naive != k0)
+
+
+# --- widths: reference known-answer test, test/lbl/test_broadening.py:147-151 --
+# Line nu=2150.856008 cm-1 of HITRAN CO (test/files/hitran_co_3iso_2000_2300cm.par:349):
+# gamma_air 0.0748, gamma_self 0.082, n_air 0.75, CO iso 1 molar mass 27.994915; x = 1.
+@pytest.mark.parametrize("T,p_bar,fwhm_l,fwhm_g", [(3000, 1, 0.02849411, 0.01594728),
+                                                   (300, 1, 0.16023415, 0.00504297),
+                                                   (3000, 0.01, 0.00028494, 0.01594728)])
+def test_fwhm_known_answers(T, p_bar, fwhm_l, fwhm_g):
+    nu = 2150.856008
+    wl = 2 * o.pressure_hwhm(0.0748, 0.082, 0.75, None, p_bar, 1.0, T, 296.0)
+    wg = 2 * o.doppler_hwhm(nu, 27.994915, T)
+    assert np.isclose(wl, fwhm_l) and np.isclose(wg, fwhm_g)
+
+
+def test_fixture_line_349_is_the_kat_line():
+    c = hitran.read_par("tests/golden/hitran_co_3iso_2000_2300cm.par")
+    i = np.argmin(np.abs(c["nu0"] - 2150.856008))
+    assert c["nu0"][i] == 2150.856008 and c["iso"][i] == 1
+    assert (c["g_air"][i], c["g_self"][i], c["n_air"][i], c["delta"][i]) == (0.0748, 0.082, 0.75, -0.0024)
+
+
+def test_pressure_hwhm_air_and_self_terms():
+    # x = 0: pure air term; exponents act on Tref/T
+    g = o.pressure_hwhm(0.07, 0.1, 0.7, 0.5, o.ATM_PER_BAR, 0.0, 592.0, 296.0)
+    assert np.isclose(g, 0.07 * 0.5 ** 0.7)
+    g = o.pressure_hwhm(0.07, 0.1, 0.7, 0.5, o.ATM_PER_BAR, 1.0, 592.0, 296.0)
+    assert np.isclose(g, 0.1 * 0.5 ** 0.5)
+
+
+# --- S(T): lbl/base.py:2664-2674 ---------------------------------------------------
+def test_linestrength_identity_with_einstein_form():
+    """S(T) from S(Tref) equals the Einstein-A form of the reference GPU
+    (cmdFillLDM.comp:126): S0 exp(-c2 E/T)(1-exp(-c2 nu/T))/Q(T), when
+    S(Tref) is defined from the same S0."""
+    rng = np.random.default_rng(3)
+    nu = rng.uniform(500, 4000, 50)
+    E = rng.uniform(0, 10000, 50)
+    S0 = 10 ** rng.uniform(-25, -18, 50)
+    Q = lambda T: 0.8 * T ** 1.2
+    c2, Tref = o.hc_k, 296.0
+    S_ref = S0 * np.exp(-c2 * E / Tref) * (1 - np.exp(-c2 * nu / Tref)) / Q(Tref)
+    for T in (200.0, 296.0, 1500.0, 3000.0):
+        got = o.linestrength_eq(S_ref, E, nu, T, Tref, Q(Tref) / Q(T))
+        want = S0 * np.exp(-c2 * E / T) * (1 - np.exp(-c2 * nu / T)) / Q(T)
+        np.testing.assert_allclose(got, want, rtol=1e-12)
+    np.testing.assert_allclose(o.linestrength_eq(S_ref, E, nu, Tref, Tref, 1.0), S_ref, rtol=1e-15)
+
+
+# --- LDM axes: lbl/broadening.py:1823-1837 -------------------------------------------
+def test_w_axis():
+    w = np.array([0.01, 0.02, 0.035])
+    ax = o.init_w_axis(w, 0.2)
+    assert ax[0] == 0.01 and ax[-1] >= 0.035 + 1e-4 and ax[-2] < 0.035 + 1e-4
+    np.testing.assert_allclose(ax[1:] / ax[:-1], np.exp(0.2), rtol=1e-14)
+
+
+# --- Voigt FT: closed form vs numerical transform of scipy's Voigt profile ----------
+@pytest.mark.parametrize("hG,hL", [(0.02, 0.01), (0.005, 0.03), (0.01, 1e-4)])
+def test_voigt_ft_is_transform_of_voigt_profile(hG, hL):
+    sigma = hG / np.sqrt(2 * np.log(2))
+    x = np.linspace(-4000 * max(hG, hL), 4000 * max(hG, hL), 2_000_001)  # wing loss ~ 2/(pi*4000)
+    prof = voigt_profile(x, sigma, hL)
+    dx = x[1] - x[0]
+    for f in (0.0, 5.0, 20.0, 60.0):
+        num = np.sum(prof * np.cos(2 * np.pi * f * x)) * dx
+        assert abs(num - o.voigt_FT(f, hG, hL)) < 2e-3
+
+
+def test_folding_terms_brute_force():
+    """The folded template equals the explicit alias sum
+    V(x) + sum_{n odd} V((n+1)/(2 ws) - x) + sum_{n even} V(n/(2 ws) + x)."""
+    L, ws = 64, 0.05
+    wG, wL = np.array([0.02]), np.array([0.01])
+    T = o.lineshape_templates(L, ws, wG, wL)[0, 0]
+    x = np.fft.rfftfreq(L, ws)
+    nf = o.fold_count(ws, wG[0] / 2, wL[0] / 2)
+    assert nf > 0
+    tot = o.voigt_FT(x, 0.01, 0.005)
+    for n in range(1, nf + 1):
+        xa = (n + 1) / (2 * ws) - x if n % 2 else n / (2 * ws) + x
+        tot = tot + o.voigt_FT(xa, 0.01, 0.005)
+    np.testing.assert_allclose(T, tot / tot[0], rtol=1e-12)
+
+
+# --- scatter and FFT apply: brute force on tiny inputs -------------------------------
+def test_ldm_scatter_brute_force():
+    rng = np.random.default_rng(4)
+    n, L, NG, NL = 40, 16, 3, 4
+    k0 = rng.integers(0, 7, n); l0 = rng.integers(0, NG - 1, n); m0 = rng.integers(0, NL - 1, n)
+    tv, tg, tl, S = rng.random(n), rng.random(n), rng.random(n), rng.random(n)
+    got = o.ldm_scatter(L, NG, NL, k0, k0 + 1, tv, l0, l0 + 1, tg, m0, m0 + 1, tl, S)
+    want = np.zeros((L, NG, NL))
+    for i in range(n):
+        for a, wa in ((0, 1 - tv[i]), (1, tv[i])):
+            for b, wb in ((0, 1 - tg[i]), (1, tg[i])):
+                for d, wd in ((0, 1 - tl[i]), (1, tl[i])):
+                    want[k0[i] + a, l0[i] + b, m0[i] + d] += S[i] * wa * wb * wd
+    np.testing.assert_allclose(got, want, rtol=1e-13)
+
+
+def test_ldm_apply_fft_is_circular_convolution():
+    rng = np.random.default_rng(5)
+    L, NG, NL, Nv, ws = 24, 2, 2, 12, 0.1
+    LDM = np.zeros((L, NG, NL))
+    LDM[:Nv] = rng.random((Nv, NG, NL))
+    tmpl = rng.random((NG, NL, L // 2 + 1))
+    got = o.ldm_apply_fft(LDM, tmpl, Nv, ws)
+    want = np.zeros(Nv)
+    for l in range(NG):
+        for m in range(NL):
+            kern = np.fft.irfft(tmpl[l, m], L)  # real symmetric kernel
+            for i in range(Nv):
+                want[i] += sum(LDM[j, l, m] * kern[(i - j) % L] for j in range(L))
+    np.testing.assert_allclose(got, want / ws, rtol=1e-10, atol=1e-12)
+
+
+# --- end to end --------------------------------------------------------------------
+def test_single_line_on_grid_point_is_voigt_profile():
+    """One line exactly on a grid point sits on the LDM axis nodes, so the
+    synthesised spectrum is the sampled (periodised) Voigt profile."""
+    wmin, wmax, ws = 1995.0, 2005.0, 0.001
+    w = o.wavenumber_grid(wmin, wmax, ws)
+    nu = w[5000]
+    lines = dict(nu0=np.array([nu]), S_ref=np.array([1e-20]), El=np.array([0.0]), g_air=np.array([0.05]),
+                 g_self=np.array([0.05]), n_air=np.array([0.7]), molar_mass=np.array([0.0, 28.0]),
+                 Q=lambda i, T: 1.0)
+    T, p, x = 296.0, 1.01325, 0.5
+    a, det = o.eq_abscoeff(lines, wmin, wmax, ws, T, p, x, dxG=0.2, dxL=0.2, return_details=True)
+    assert det["ki0"][0] == 5000 and det["tvi"][0] == 0.0 and det["li0"][0] == 0 and det["mi0"][0] == 0
+    hG, hL = det["wG_dat"][0] / 2, det["wL_dat"][0] / 2
+    dens = o.number_density(x, p, T)
+    S = det["S"][0]
+    prof = sum(voigt_profile(w - nu + m * 2 * w.size * ws, hG / np.sqrt(2 * np.log(2)), hL) for m in (-1, 0, 1))
+    want = S * dens * prof
+    assert np.max(np.abs(a - want)) / want.max() < 1e-5
+
+
+def test_area_and_linearity():
+    lines, c = workloads.workload("co_5k")
+    d = lines.as_dict()
+    a, det = o.eq_abscoeff(d, c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction, c.dxG, c.dxL,
+                           return_details=True)
+    dens = o.number_density(c.mole_fraction, c.pressure_bar, c.Tgas)
+    area = a.sum() * c.wstep
+    assert abs(area / (det["S"].sum() * dens) - 1) < 2e-3  # Lorentz wings beyond the grid
+    d2 = dict(d, S_ref=2 * d["S_ref"])
+    a2 = o.eq_abscoeff(d2, c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction, c.dxG, c.dxL)
+    np.testing.assert_allclose(a2, 2 * a, rtol=1e-12, atol=1e-12 * a.max())
+
+
+def test_fft_length_changes_little_on_wide_grids():
+    lines, c = workloads.workload("co_5k")
+    d = lines.as_dict()
+    a = o.eq_abscoeff(d, c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction, c.dxG, c.dxL)
+    b = o.eq_abscoeff(d, c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction, c.dxG, c.dxL,
+                      L=401408)
+    assert o.rel_peak_error(b, a) < 1e-6
+
+
+def test_hitran_fixture_end_to_end_runs_with_shift_and_isotopes():
+    lines = hitran.line_list("tests/golden/hitran_co_3iso_2000_2300cm.par")
+    assert lines.iso.max() == 3 and np.any(lines.delta != 0)
+    a, det = o.eq_abscoeff(lines.as_dict(), 2000.0, 2300.0, 0.01, 1500.0, 2.0, 0.5, return_details=True)
+    assert det["i1"] - det["i0"] == len(lines) and np.all(np.isfinite(a)) and a.max() > 0

@@ -1,0 +1,90 @@
+"""Continuous-batching server: every result equals the same request computed
+alone (and the oracle), whatever it was batched with; admission respects
+the HBM budget."""
+import numpy as np
+import pytest
+
+from oracle import ldm_oracle as oracle
+from radis_amd import engine, serve, workloads
+
+
+def test_poisson_arrivals_rate():
+    t = serve.poisson_arrivals(200.0, 50.0, seed=3)
+    assert abs(len(t) / 50.0 - 200.0) < 10.0
+    assert np.all(np.diff(t) > 0) and t[-1] < 50.0
+
+
+def test_latency_summary():
+    recs = [serve.ServedRequest(i, "a", None, 0.0, t_done=(i + 1) * 1e-3) for i in range(100)]
+    s = serve.latency_summary(recs)
+    assert abs(s["p50_ms"] - 50.5) < 1e-9 and s["served"] == 100
+
+
+@pytest.fixture(scope="module")
+def server_setup():
+    dev = engine.Device(0)
+    dbs = {"co2": engine.LineDatabase(dev, workloads.co2_like(300_000, seed=31)),
+           "h2o": engine.LineDatabase(dev, workloads.h2o_like(300_000, seed=32, chunk=300_000))}
+    yield dev, dbs
+    for d in dbs.values():
+        d.close()
+    dev.close()
+
+
+def _conds(n, seed):
+    a = workloads.random_requests(n, 2000.0, 2500.0, seed=seed, wsteps=(0.001, 0.002, 0.005),
+                                  median_width=20.0, wmax_width=80.0)
+    b = workloads.random_requests(n, 1000.0, 4000.0, seed=seed + 1, wsteps=(0.001, 0.002, 0.005),
+                                  median_width=20.0, wmax_width=80.0)
+    return a, b
+
+
+@pytest.mark.gpu
+def test_results_same_alone_and_in_the_mix(server_setup):
+    dev, dbs = server_setup
+    a, b = _conds(24, seed=40)
+    srv = serve.SpectrumServer(dev, dbs, budget_bytes=8 << 30, max_batch=8, n_streams=3)
+    arrivals = []
+    for i in range(24):
+        arrivals.append((0.002 * i, "co2", a[i]))
+        arrivals.append((0.002 * i + 0.001, "h2o", b[i]))
+    done = srv.run_stream(arrivals, timeout=120)
+    assert len(done) == 48 and all(r.error is None for r in done)
+    for r in done[::5]:
+        db = dbs[r.db]
+        alone = engine.SpectrumBatch(db, [engine.request_from_conditions(db, r.cond)])
+        ref = alone.results(alone.run())[0]
+        alone.close()
+        assert oracle.rel_peak_error(r.result, ref) <= 1e-6, r.rid
+    for r in done[:2]:
+        lines = workloads.co2_like(300_000, seed=31) if r.db == "co2" else workloads.h2o_like(300_000, seed=32,
+                                                                                               chunk=300_000)
+        c = r.cond
+        L = dbs[r.db].request_bytes(engine.request_from_conditions(dbs[r.db], c))[1].fft_len
+        ref = oracle.eq_abscoeff(lines.as_dict(), c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar,
+                                 c.mole_fraction, c.dxG, c.dxL, L=int(L))
+        assert oracle.rel_peak_error(r.result, ref) <= 1e-4
+    s = serve.latency_summary(done)
+    print(s)
+
+
+@pytest.mark.gpu
+def test_budget_admission(server_setup):
+    dev, dbs = server_setup
+    a, _ = _conds(12, seed=50)
+    db = dbs["co2"]
+    sizes = [db.request_bytes(engine.request_from_conditions(db, c))[0] for c in a]
+    budget = int(max(sizes) * 1.5)  # at most ~one request in flight at a time
+    srv = serve.SpectrumServer(dev, dbs, budget_bytes=budget, max_batch=8, n_streams=4)
+    for c in a:
+        srv.submit("co2", c)
+    peak = 0
+    while srv.queue or srv.flights:
+        srv.step()
+        peak = max(peak, srv.in_flight_bytes)
+    assert peak <= budget
+    assert len(srv.done) == 12 and all(r.error is None for r in srv.done)
+    big = workloads.Conditions(2000.0, 2400.0, 0.001, 1000.0, 1.0, 0.5)
+    tiny = serve.SpectrumServer(dev, dbs, budget_bytes=1 << 20)
+    r = tiny.submit("co2", big)
+    assert r.error is not None and tiny.rejected == [r]
