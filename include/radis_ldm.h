@@ -88,6 +88,9 @@ typedef struct {
     int32_t flags;            /* RADIS_LDM_FP64_WIDTHS: log-width axis positions and
                                  fractions always in FP64 (default: FP32 with an exact
                                  FP64 recomputation next to axis nodes)                  */
+    int32_t fft_n1;           /* 0 = planner; else the four-step split L = fft_n1 * n2
+                                 (16 | fft_n1 <= 1024, 7-smooth cofactor)                */
+    int32_t tile_w;           /* 0 = planner; else the fill tile width (power of 2)      */
 } radis_ldm_request;
 
 #define RADIS_LDM_FP64_WIDTHS 1
@@ -154,10 +157,11 @@ int radis_ldm_batch_info(radis_ldm_batch *b, int i, radis_ldm_request_info *info
 int radis_ldm_batch_workspace_bytes(const radis_ldm_batch *b, int64_t *bytes);
 void radis_ldm_batch_destroy(radis_ldm_batch *b);
 
-/* Per-kernel device timings of the last run made with timing enabled,
- * in milliseconds, in pipeline order (axis, finalize, fill, fft_a,
- * fft_b, fft_c).  Replaces GPUApplication.get_timestamps
- * (vulkan_compute_lib.py:973-1024). */
+/* Device time per stage (axis, finalize, fill incl. tile bounds, fft_a,
+ * fft_b, fft_c) in milliseconds, averaged over the runs made with timing
+ * enabled since the previous call; each kernel is bracketed by its own
+ * start/stop events (hipExtLaunchKernelGGL) on the launch stream.
+ * Replaces GPUApplication.get_timestamps (vulkan_compute_lib.py:973-1024). */
 #define RADIS_LDM_N_STAGES 6
 int radis_ldm_batch_enable_timing(radis_ldm_batch *b, int enable);
 int radis_ldm_batch_stage_ms(radis_ldm_batch *b, float *ms_out);

@@ -74,6 +74,8 @@ class Request(ctypes.Structure):
         ("line_lo", c_double),
         ("line_hi", c_double),
         ("flags", c_int32),
+        ("fft_n1", c_int32),
+        ("tile_w", c_int32),
     ]
 
 

@@ -1,6 +1,7 @@
 // Host side of the C ABI declared in include/radis_ldm.h: device context,
 // device-resident line lists, FFT planning, per-request workspace, the job
 // table and the launch sequence of the LDM pipeline.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -209,6 +210,7 @@ struct radis_ldm_batch {
   size_t items_bytes = 0;
   int n_axis = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
   bool small_b = true;  // pass B with LDM_CAP_B tiles (every n2 <= LDM_CAP_B)
+  bool tiles_valid = false;  // fill tile bounds computed for the current layout
   size_t fill_lds = 0;
   std::vector<float*> outs;        // outputs of the last eager run
   std::vector<float*> graph_outs;  // outputs bound to the captured graph
@@ -216,7 +218,10 @@ struct radis_ldm_batch {
   bool pending[2] = {false, false};
   bool timing = false;
   bool slot_timed[2] = {false, false};
-  hipEvent_t st_ev[2][RADIS_LDM_N_STAGES + 1] = {};
+  // start/stop events of every kernel of a timed run (tile bounds counted in the fill stage)
+  static constexpr int N_KERNELS = RADIS_LDM_N_STAGES + 1;
+  hipEvent_t st_ev[2][2 * N_KERNELS] = {};
+  bool ev_used[2][N_KERNELS] = {};
   double stage_sum[RADIS_LDM_N_STAGES] = {0};
   int stage_count = 0;
   hipStream_t own_stream = nullptr;
@@ -253,12 +258,13 @@ bool radices_of(int n, std::vector<int>& r) {
 
 // Search L = n1*n2 >= 2*N_v (or == want) with 16 | n1 <= 1024 and
 // 16 | n2 <= n2_max, 7-smooth cofactors; smallest L, then n1 closest to 512.
-int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2) {
+int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int only_n1 = 0) {
   static const std::vector<int> c1 = smooth16_list(N1_MAX);
   static const std::vector<int> c2 = smooth16_list(N2_MAX);
   int64_t bestL = -1;
   double bestScore = 1e300;
   for (int n1 : c1) {
+    if (only_n1 && n1 != only_n1) continue;
     int n2 = 0;
     if (want > 0) {
       if (want % n1) continue;
@@ -281,12 +287,12 @@ int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2) {
   return bestL;
 }
 
-int plan_fft(int64_t nv, int64_t want, FftPlan& out) {
+int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0) {
   const int64_t need = 2 * nv;
   if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);
   int a1 = 0, a2 = 0, s1 = 0, s2 = 0;
-  const int64_t any = search_fft(need, want, N2_MAX, a1, a2);
-  const int64_t small = search_fft(need, want, LDM_CAP_B, s1, s2);  // pass B with many small tiles
+  const int64_t any = search_fft(need, want, N2_MAX, a1, a2, only_n1);
+  const int64_t small = search_fft(need, want, LDM_CAP_B, s1, s2, only_n1);  // pass B with many small tiles
   if (any < 0) {
     if (want > 0)
       return fail(RADIS_LDM_ERANGE, "fft_len %lld is not n1*n2 with 16|n1<=%d, 16|n2<=%d and 7-smooth cofactors",
@@ -402,6 +408,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   std::memset(&J, 0, sizeof J);
   J.wmin = r.wmin;
   J.delta_w = R.delta_w;
+  J.inv_delta_w = 1.0 / R.delta_w;
   J.wstep = r.wstep;
   J.nv = R.nv;
   J.v = R.nv + 1;
@@ -478,6 +485,7 @@ void free_layout(radis_ldm_batch* b) {
 // Allocate workspace, twiddle tables and work-item lists for every request.
 int build_layout(radis_ldm_batch* b) {
   free_layout(b);
+  b->tiles_valid = false;
   // --- workspace
   size_t total = 0;
   size_t fill_lds = 0;
@@ -485,7 +493,7 @@ int build_layout(radis_ldm_batch* b) {
   for (const Req& R : b->reqs)
     if (R.plan.n2 > LDM_CAP_B) b->small_b = false;
   for (Req& R : b->reqs) {
-    R.tile_w = choose_tile(R.m_cap);
+    R.tile_w = R.r.tile_w ? R.r.tile_w : choose_tile(R.m_cap);
     if ((size_t)R.m_cap * (R.tile_w + 1) * 4 > (size_t)FILL_LDS_TARGET)
       return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit the fill tile (max %d rows)", R.m_cap,
                   FILL_LDS_TARGET / (4 * MIN_TILE_W));
@@ -600,7 +608,11 @@ int plan_one(const radis_ldm_db& db, const radis_ldm_request& rq, Req& R) {
     if (rq.n_v < 2 || rq.n_v > 1000000000) return fail(RADIS_LDM_EINVAL, "n_v %lld out of range", (long long)rq.n_v);
     R.nv = rq.n_v;
   }
-  rc = plan_fft(R.nv, rq.fft_len, R.plan);
+  if (rq.fft_n1 && (rq.fft_n1 < 16 || rq.fft_n1 > N1_MAX || rq.fft_n1 % 16))
+    return fail(RADIS_LDM_EINVAL, "fft_n1 %d: need 16 | fft_n1 <= %d", rq.fft_n1, N1_MAX);
+  if (rq.tile_w && (rq.tile_w < MIN_TILE_W || rq.tile_w > MAX_TILE_W || (rq.tile_w & (rq.tile_w - 1))))
+    return fail(RADIS_LDM_EINVAL, "tile_w %d: need a power of 2 in [%d, %d]", rq.tile_w, MIN_TILE_W, MAX_TILE_W);
+  rc = plan_fft(R.nv, rq.fft_len, R.plan, rq.fft_n1);
   if (rc) return rc;
   const bool lw = rq.line_hi > rq.line_lo;
   const double lo = lw ? rq.line_lo : rq.wmin, hi = lw ? rq.line_hi : rq.wmax;
@@ -623,42 +635,56 @@ int download_results(radis_ldm_batch* b, hipStream_t s, int slot) {
   return 0;
 }
 
+// Launch one kernel; in a timed run with per-dispatch start/stop events,
+// which the runtime stamps on the kernel's own packet.
+template <typename K, typename... Args>
+void launch(radis_ldm_batch* b, bool timed, int slot, int kid, K kernel, dim3 grid, dim3 block, size_t shmem,
+            hipStream_t s, Args... args) {
+  if (timed) {
+    hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, s, b->st_ev[slot][2 * kid],
+                          b->st_ev[slot][2 * kid + 1], 0, args...);
+    b->ev_used[slot][kid] = true;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+  }
+}
+
+// stage of each kernel id: axis, finalize, tile bounds (fill), fill, fft_a, fft_b, fft_c
+const int KERNEL_STAGE[RADIS_LDM_N_STAGES + 1] = {0, 1, 2, 2, 3, 4, 5};
+
 // The kernel sequence of one run (what a graph captures).
-int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
+int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool capturing = false) {
   const radis_ldm_db& db = *b->db;
   const int4* items = b->items_d;
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][0], s));
-  if (b->n_axis) hipLaunchKernelGGL(k_axis, dim3(b->n_axis), dim3(256), 0, s, b->jobs_d, items, db.dev);
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][1], s));
-  hipLaunchKernelGGL(k_finalize, dim3(b->n), dim3(64), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][2], s));
+  for (int k = 0; k < radis_ldm_batch::N_KERNELS; ++k) b->ev_used[slot][k] = false;
+  const LdmJob* cj = b->jobs_d;
+  const LdmAxes* ca = b->axes_d;
+  if (b->n_axis) launch(b, timed, slot, 0, k_axis, dim3(b->n_axis), dim3(256), 0, s, b->jobs_d, items, db.dev);
+  launch(b, timed, slot, 1, k_finalize, dim3(b->n), dim3(64), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
   items += b->n_axis;
   if (b->n_fill) {
-    hipLaunchKernelGGL(k_tile_bounds, dim3((b->n_fill + 255) / 256), dim3(256), 0, s, b->jobs_d, items, b->n_fill,
-                       db.dev);
-    hipLaunchKernelGGL(k_fill, dim3(b->n_fill), dim3(LDM_FILL_T), b->fill_lds, s, (const LdmJob*)b->jobs_d,
-                       (const LdmAxes*)b->axes_d, items, db.dev);
+    // Tile line bounds depend on the grid and, through the pressure-shift
+    // margin, on p; without a shift column they are fixed per layout.
+    if (db.has_delta || !b->tiles_valid) {
+      launch(b, timed, slot, 2, k_tile_bounds, dim3((b->n_fill + 255) / 256), dim3(256), 0, s, b->jobs_d, items,
+             b->n_fill, db.dev);
+      // a captured launch has not run yet: only an eager run validates the bounds
+      if (!capturing) b->tiles_valid = !db.has_delta;
+    }
+    launch(b, timed, slot, 3, k_fill, dim3(b->n_fill), dim3(LDM_FILL_T), b->fill_lds, s, cj, ca, items, db.dev);
   }
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][3], s));
   items += b->n_fill;
-  if (b->n_a)
-    hipLaunchKernelGGL(k_fft_a, dim3(b->n_a), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d,
-                       (const LdmAxes*)b->axes_d, items, b->n_a);
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][4], s));
+  if (b->n_a) launch(b, timed, slot, 4, k_fft_a, dim3(b->n_a), dim3(LDM_FFT_T), 0, s, cj, ca, items, b->n_a);
   items += b->n_a;
   if (b->n_b) {
     if (b->small_b)
-      hipLaunchKernelGGL((k_fft_b<LDM_CAP_B, LDM_FFT_T_B>), dim3(b->n_b), dim3(LDM_FFT_T_B), 0, s,
-                         (const LdmJob*)b->jobs_d, (const LdmAxes*)b->axes_d, items);
+      launch(b, timed, slot, 5, k_fft_b<LDM_CAP_B, LDM_FFT_T_B>, dim3(b->n_b), dim3(LDM_FFT_T_B), 0, s, cj, ca,
+             items);
     else
-      hipLaunchKernelGGL((k_fft_b<LDM_CAP, LDM_FFT_T>), dim3(b->n_b), dim3(LDM_FFT_T), 0, s,
-                         (const LdmJob*)b->jobs_d, (const LdmAxes*)b->axes_d, items);
+      launch(b, timed, slot, 5, k_fft_b<LDM_CAP, LDM_FFT_T>, dim3(b->n_b), dim3(LDM_FFT_T), 0, s, cj, ca, items);
   }
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][5], s));
   items += b->n_b;
-  if (b->n_c)
-    hipLaunchKernelGGL(k_fft_c, dim3(b->n_c), dim3(LDM_FFT_T), 0, s, (const LdmJob*)b->jobs_d, items, b->n_c);
-  if (timed) HIPCHK(hipEventRecord(b->st_ev[slot][6], s));
+  if (b->n_c) launch(b, timed, slot, 6, k_fft_c, dim3(b->n_c), dim3(LDM_FFT_T), 0, s, cj, items, b->n_c);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -677,10 +703,11 @@ int wait_slot(radis_ldm_batch* b, int slot) {
   HIPCHK(hipEventSynchronize(b->ev_done[slot]));
   b->pending[slot] = false;
   if (b->slot_timed[slot]) {
-    for (int k = 0; k < RADIS_LDM_N_STAGES; ++k) {
+    for (int k = 0; k < radis_ldm_batch::N_KERNELS; ++k) {
+      if (!b->ev_used[slot][k]) continue;
       float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, b->st_ev[slot][k], b->st_ev[slot][k + 1]));
-      b->stage_sum[k] += ms;
+      HIPCHK(hipEventElapsedTime(&ms, b->st_ev[slot][2 * k], b->st_ev[slot][2 * k + 1]));
+      b->stage_sum[KERNEL_STAGE[k]] += ms;
     }
     b->stage_count++;
     b->slot_timed[slot] = false;
@@ -866,7 +893,7 @@ int radis_ldm_request_bytes(const radis_ldm_db* db, const radis_ldm_request* req
   int rc = plan_one(*db, *req, R);
   if (rc) return rc;
   R.m_cap = R.m_need;
-  R.tile_w = choose_tile(R.m_cap);
+  R.tile_w = req->tile_w ? req->tile_w : choose_tile(R.m_cap);
   const int64_t V = R.nv + 1;
   const int64_t stride = (int64_t)align_up((size_t)V, 64);
   const int64_t tiles = (V + R.tile_w - 1) / R.tile_w;
@@ -941,7 +968,8 @@ int radis_ldm_batch_set_request(radis_ldm_batch* b, int i, const radis_ldm_reque
   Req& R = b->reqs[i];
   if (req->wmin != R.r.wmin || req->wmax != R.r.wmax || req->wstep != R.r.wstep ||
       (req->fft_len != 0 && req->fft_len != R.plan.L) || req->n_v != R.r.n_v ||
-      req->line_lo != R.r.line_lo || req->line_hi != R.r.line_hi)
+      req->line_lo != R.r.line_lo || req->line_hi != R.r.line_hi || req->fft_n1 != R.r.fft_n1 ||
+      req->tile_w != R.r.tile_w)
     return fail(RADIS_LDM_EINVAL, "set_request cannot change the grid; create a new batch");
   int rc = validate_request(*req);
   if (rc) return rc;
@@ -997,7 +1025,7 @@ int radis_ldm_batch_graph_capture(radis_ldm_batch* b, float* const* out_dev, voi
     s = b->own_stream;
   }
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  rc = launch_kernels(b, s, false, 0);
+  rc = launch_kernels(b, s, false, 0, true);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(s, &g);
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }

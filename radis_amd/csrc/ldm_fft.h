@@ -181,8 +181,8 @@ struct LdsSize {
 };
 
 template <int R, bool FWD, int CAP, int T>
-__device__ __noinline__ void stockham_stage(float2* __restrict__ buf, int N, int bshift, int Ns,
-                                            const float2* __restrict__ tw) {
+__device__ __forceinline__ void stockham_stage_body(float2* __restrict__ buf, int N, int bshift, int Ns,
+                                                    const float2* __restrict__ tw) {
   constexpr int MAXIT = (CAP + R * T - 1) / (R * T);
   const int tid = threadIdx.x;
   const int NR = N / R;
@@ -226,6 +226,20 @@ __device__ __noinline__ void stockham_stage(float2* __restrict__ buf, int N, int
     }
   }
   __syncthreads();
+}
+
+// One out-of-line function per radix keeps the register allocation of the
+// radix variants separate (inlined together they spill).
+template <int R, bool FWD, int CAP, int T>
+__device__ __noinline__ void stockham_stage_call(float2* __restrict__ buf, int N, int bshift, int Ns,
+                                                 const float2* __restrict__ tw) {
+  stockham_stage_body<R, FWD, CAP, T>(buf, N, bshift, Ns, tw);
+}
+
+template <int R, bool FWD, int CAP, int T>
+__device__ __forceinline__ void stockham_stage(float2* __restrict__ buf, int N, int bshift, int Ns,
+                                               const float2* __restrict__ tw) {
+  stockham_stage_call<R, FWD, CAP, T>(buf, N, bshift, Ns, tw);
 }
 
 // Full transform of the B = 2^bshift columns in buf (forward: exp(-2 pi i nk/N)),

@@ -18,7 +18,7 @@
 #define LDM_CAP_B 2048     // complex elements held by a small pass B workgroup
 #define LDM_FFT_T_B 256    // threads of a small pass B workgroup
 #define LDM_FILL_T 256     // threads of a fill workgroup
-#define LDM_AX_CHUNK 2048  // lines per axis-reduction workgroup
+#define LDM_AX_CHUNK 4096  // lines per axis-reduction workgroup (16 per lane)
 #define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
 #define LDM_NFOLD_MAX 100000
 
@@ -43,6 +43,7 @@ struct LdmDb {
 
 struct LdmJob {
   double wmin, delta_w, wstep;   // grid w[i] = wmin + i*delta_w
+  double inv_delta_w;            // 1/delta_w (first guess of a grid index only)
   long long nv, v;               // N_v and V = N_v + 1 (written LDM length)
   long long L;                   // FFT length = n1*n2
   long long i0, i1;              // window line range

@@ -31,19 +31,30 @@
 //   x < w[0] -> 0; x >= w[nv-1] -> nv-1; w[j] == x -> j;
 //   else (1/(w[j+1]-w[j]))*(x-w[j]) + j with w[j] <= x < w[j+1].
 // radis/lbl/broadening.py:2151-2154, 2253.
-__device__ __noinline__ double grid_position(double x, double wmin, double delta, long long nv) {
+__device__ __forceinline__ double grid_position(double x, double wmin, double delta, double inv_delta,
+                                                long long nv) {
 #pragma clang fp contract(off)
   const double wlast = wmin + (double)(nv - 1) * delta;
   if (x < wmin) return 0.0;
   if (!(x < wlast)) return (double)(nv - 1);
-  long long j = (long long)floor((x - wmin) / delta);
-  if (j < 0) j = 0;
-  if (j > nv - 2) j = nv - 2;
-  while (j > 0 && wmin + (double)j * delta > x) --j;
-  while (j < nv - 2 && wmin + (double)(j + 1) * delta <= x) ++j;
-  const double wj = wmin + (double)j * delta;
+  // First guess: (x - wmin)/delta has a relative error of a few ulp, i.e.
+  // ~1e-9 grid steps for grids below 1e7 points, so the largest j with
+  // w[j] <= x is the guess or one of its neighbours; one exact comparison
+  // each way settles it.
+  long long j = (long long)floor((x - wmin) * inv_delta);
+  j = j < 0 ? 0 : (j > nv - 2 ? nv - 2 : j);
+  double wj = wmin + (double)j * delta;
+  if (wj > x && j > 0) {
+    --j;
+    wj = wmin + (double)j * delta;
+  }
+  double wj1 = wmin + (double)(j + 1) * delta;
+  if (wj1 <= x && j < nv - 2) {
+    ++j;
+    wj = wj1;
+    wj1 = wmin + (double)(j + 1) * delta;
+  }
   if (wj == x) return (double)j;
-  const double wj1 = wmin + (double)(j + 1) * delta;
   const double slope = 1.0 / (wj1 - wj);
   return slope * (x - wj) + (double)j;
 }
@@ -108,21 +119,51 @@ __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const i
   long long e = a + LDM_AX_CHUNK;
   if (e > J.i1) e = J.i1;
   unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
-  // FP32 filter: a line can only be this thread's exact (FP64) extreme if its
-  // FP32 width is within the FP32 error bound of the running FP32 extreme.
-  float fmin = 3.0e38f, fmax = 0.0f;
-  for (long long i = a + threadIdx.x; i < e; i += blockDim.x) {
-    const float ga = db.g_air[i], gs = db.g_self[i], na = db.n_air[i];
-    const float ns = db.n_self ? db.n_self[i] : na;
-    const float wf = fwhm_lorentz_f32(J, ga, gs, na, ns);
-    const bool cmin = wf <= fmin * (1.0f + 1e-5f);
-    const bool cmax = wf >= fmax * (1.0f - 1e-5f);
-    if (cmin || cmax) {
-      const unsigned long long bl = double_to_bits(fwhm_lorentz(J, ga, gs, na, ns));
-      if (cmin) { lmin = bl < lmin ? bl : lmin; fmin = wf < fmin ? wf : fmin; }
-      if (cmax) { lmax = bl > lmax ? bl : lmax; fmax = wf > fmax ? wf : fmax; }
+  // FP32 widths of this lane's lines stay in registers; only the FP32 argmin
+  // and argmax are re-evaluated in FP64.  If another of the lane's lines is
+  // within the FP32 error bound of an extreme (a near tie), every line of the
+  // lane is evaluated in FP64 (rare), so the exact FP64 extremes are found.
+  constexpr int NPER = LDM_AX_CHUNK / 256;
+  float wf[NPER];
+  float fmin = 3.0e38f, fmax = -1.0f;
+  int kmin = -1, kmax = -1;
+#pragma unroll
+  for (int k = 0; k < NPER; ++k) {
+    const long long i = a + threadIdx.x + (long long)k * 256;
+    wf[k] = -1.0f;
+    if (i < e) {
+      const float na = db.n_air[i];
+      wf[k] = fwhm_lorentz_f32(J, db.g_air[i], db.g_self[i], na, db.n_self ? db.n_self[i] : na);
+      if (wf[k] < fmin) { fmin = wf[k]; kmin = k; }
+      if (wf[k] > fmax) { fmax = wf[k]; kmax = k; }
     }
-    if (!J.single_iso) {
+  }
+  bool tie = false;
+#pragma unroll
+  for (int k = 0; k < NPER; ++k) {
+    if (wf[k] >= 0.0f && k != kmin && wf[k] <= fmin * (1.0f + 2e-5f)) tie = true;
+    if (wf[k] >= 0.0f && k != kmax && wf[k] >= fmax * (1.0f - 2e-5f)) tie = true;
+  }
+  auto exact = [&](int k) {
+    const long long i = a + threadIdx.x + (long long)k * 256;
+    const double na = db.n_air[i];
+    return double_to_bits(fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, db.n_self ? (double)db.n_self[i] : na));
+  };
+  if (kmin >= 0) {
+    if (!tie) {
+      lmin = exact(kmin);
+      lmax = exact(kmax);
+    } else {
+      for (int k = 0; k < NPER; ++k)
+        if (wf[k] >= 0.0f) {
+          const unsigned long long bl = exact(k);
+          lmin = bl < lmin ? bl : lmin;
+          lmax = bl > lmax ? bl : lmax;
+        }
+    }
+  }
+  if (!J.single_iso) {
+    for (long long i = a + threadIdx.x; i < e; i += blockDim.x) {
       const int iso = db.iso ? db.iso[i] : 1;
       const unsigned long long bg = double_to_bits(fwhm_gauss(db.nu0[i], J.dopp[iso]));
       gmin = bg < gmin ? bg : gmin;
@@ -178,11 +219,13 @@ __global__ __launch_bounds__(64) void k_finalize(LdmJob* __restrict__ jobs, LdmA
   LdmJob& J = jobs[blockIdx.x];
   LdmAxes& A = axes[blockIdx.x];
   __shared__ double wgax[LDM_MAX_AX], wlax[LDM_MAX_AX];
+  __shared__ double s_wg_min, s_wl_min;
   __shared__ int s_ng, s_nl, s_status;
   if (threadIdx.x == 0) {
     int status = 0, ng = 0, nl = 0;
+    double wg_min = 0, wl_min = 0;
     if (J.i1 > J.i0) {
-      double wg_min, wg_max;
+      double wg_max;
       if (J.single_iso) {
         wg_min = fwhm_gauss(db.nu0[J.i0], J.dopp[1]);
         wg_max = fwhm_gauss(db.nu0[J.i1 - 1], J.dopp[1]);
@@ -190,41 +233,44 @@ __global__ __launch_bounds__(64) void k_finalize(LdmJob* __restrict__ jobs, LdmA
         wg_min = bits_to_double(J.wg_min_bits);
         wg_max = bits_to_double(J.wg_max_bits);
       }
-      const double wl_min = bits_to_double(J.wl_min_bits);
+      wl_min = bits_to_double(J.wl_min_bits);
       const double wl_max = bits_to_double(J.wl_max_bits);
       if (!(wg_min > 0.0) || !(wl_min > 0.0)) {
         status |= LDM_ST_WIDTH;
       } else {
-        const double gmx = wg_max + 1e-4, lmx = wl_max + 1e-4;
-        ng = (int)ceil((log(gmx) - log(wg_min)) / J.dxG) + 1;
-        nl = (int)ceil((log(lmx) - log(wl_min)) / J.dxL) + 1;
-        if (ng > LDM_MAX_AX || nl > LDM_MAX_AX || ng < 2 || nl < 2) {
-          status |= LDM_ST_AXIS;
-        } else {
-          for (int i = 0; i < ng; ++i) {
-            wgax[i] = wg_min * exp(J.dxG * (double)i);
-            A.laG[i] = log(wgax[i]);
-          }
-          for (int i = 0; i < nl; ++i) {
-            wlax[i] = wl_min * exp(J.dxL * (double)i);
-            A.laL[i] = log(wlax[i]);
-          }
-          for (int i = 0; i + 1 < ng; ++i) A.slG[i] = 1.0 / (A.laG[i + 1] - A.laG[i]);
-          for (int i = 0; i + 1 < nl; ++i) A.slL[i] = 1.0 / (A.laL[i + 1] - A.laL[i]);
-          J.laG0_f = (float)A.laG[0];
-          J.laL0_f = (float)A.laL[0];
-          J.inv_dxG_f = (float)(1.0 / J.dxG);
-          J.inv_dxL_f = (float)(1.0 / J.dxL);
-          if (ng * nl > J.m_cap) status |= LDM_ST_CAP;
-        }
+        // N = ceil((log(w_max + 1e-4) - log(w_min)) / dx) + 1, radis/lbl/broadening.py:1835-1836
+        ng = (int)ceil((log(wg_max + 1e-4) - log(wg_min)) / J.dxG) + 1;
+        nl = (int)ceil((log(wl_max + 1e-4) - log(wl_min)) / J.dxL) + 1;
+        if (ng > LDM_MAX_AX || nl > LDM_MAX_AX || ng < 2 || nl < 2) status |= LDM_ST_AXIS;
+        else if (ng * nl > J.m_cap) status |= LDM_ST_CAP;
       }
     }
     if (status) { ng = 0; nl = 0; }
+    s_ng = ng; s_nl = nl; s_status = status;
+    s_wg_min = wg_min; s_wl_min = wl_min;
     A.ng = ng;
     A.nl = nl;
     A.m = ng * nl;
     A.p = (ng * nl + 1) / 2;
-    s_ng = ng; s_nl = nl; s_status = status;
+  }
+  __syncthreads();
+  // axis nodes w_min * exp(dx * i) and their logs, radis/lbl/broadening.py:1837
+  for (int i = threadIdx.x; i < s_ng; i += blockDim.x) {
+    wgax[i] = s_wg_min * exp(J.dxG * (double)i);
+    A.laG[i] = log(wgax[i]);
+  }
+  for (int i = threadIdx.x; i < s_nl; i += blockDim.x) {
+    wlax[i] = s_wl_min * exp(J.dxL * (double)i);
+    A.laL[i] = log(wlax[i]);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i + 1 < s_ng; i += blockDim.x) A.slG[i] = 1.0 / (A.laG[i + 1] - A.laG[i]);
+  for (int i = threadIdx.x; i + 1 < s_nl; i += blockDim.x) A.slL[i] = 1.0 / (A.laL[i + 1] - A.laL[i]);
+  if (threadIdx.x == 0 && s_ng > 0) {
+    J.laG0_f = (float)A.laG[0];
+    J.laL0_f = (float)A.laL[0];
+    J.inv_dxG_f = (float)(1.0 / J.dxG);
+    J.inv_dxL_f = (float)(1.0 / J.dxL);
   }
   __syncthreads();
   const int ng = s_ng, nl = s_nl;
@@ -308,6 +354,17 @@ __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, 
 // the FP64 ones.
 #define LDM_TIE_EPS 2e-4f
 
+// s * exp(x) rounded to FP32 for |x| up to ~700: the exponent x (FP64) is
+// reduced in FP64, x = n ln2 + r with |r| <= ln2/2, so the FP32 exp of r
+// (relative error ~1e-7) and the exact power of two keep the full relative
+// accuracy that an FP32 exponent of magnitude ~100 would lose.
+__device__ __forceinline__ float boltzmann_scale(double s, double x) {
+#pragma clang fp contract(off)
+  const double n = rint(x * 1.4426950408889634);
+  const double r = (x - n * 0.6931471805599453) - n * 2.3190468138462996e-17;
+  return (float)ldexp(s * (double)__expf((float)r), (int)n);
+}
+
 struct LineTerms {
   long long k0;
   int l0, m0;
@@ -328,7 +385,7 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
 #pragma clang fp contract(off)
     nu = nu0 + ((double)db.delta[i] * J.p_atm);
   }
-  const double pos = grid_position(nu, J.wmin, J.delta_w, J.nv);
+  const double pos = grid_position(nu, J.wmin, J.delta_w, J.inv_delta_w, J.nv);
   t.k0 = (long long)pos;
   t.tv = (float)(pos - (double)t.k0);
   const int iso = db.iso ? db.iso[i] : 1;
@@ -360,9 +417,9 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
   }
   if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
   if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
-  const double s64 = (double)db.S_ref[i] * J.qratio[iso] * exp(J.c_boltz * (double)db.El[i]) * J.i_scale;
+  const double s64 = (double)db.S_ref[i] * J.qratio[iso] * J.i_scale;
   const float stim = expm1f(J.c_stim_T * nuf) / expm1f(J.c_stim_R * nuf);
-  t.I = (float)s64 * stim;
+  t.I = boltzmann_scale(s64, J.c_boltz * (double)db.El[i]) * stim;
 }
 
 // ---------------------------------------------------------------------------
@@ -372,11 +429,10 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
 // fall in the tile into LDS with ds_add_f32.  The tile is written to HBM
 // once, so the LDM needs no clearing and no global atomics.  Row index
 // r = l*N_L + m; LDS row stride W+1 so rows fall on different banks.
-// The tile's lines are split into 16 contiguous chunks, one per group of 16
-// lanes: a wave works on 4 distant parts of the tile at once, which keeps
-// its 64 atomics of one corner off the same LDS addresses.
+// Each lane walks its own contiguous run of the tile's nu-sorted lines: the
+// 64 lanes of an atomic instruction then target 64 distant grid points, so
+// same-address serialisation in LDS is rare.
 // ---------------------------------------------------------------------------
-#define LDM_FILL_GROUP 16
 
 __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
@@ -400,32 +456,34 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   }
   __syncthreads();
   const long long lo = J.tiles[2 * it.y], hi = J.tiles[2 * it.y + 1];
-  const int ngroups = LDM_FILL_T / LDM_FILL_GROUP;
-  const int gid = threadIdx.x / LDM_FILL_GROUP, lane = threadIdx.x % LDM_FILL_GROUP;
-  const long long per = (hi - lo + ngroups - 1) / ngroups;
-  const long long g0 = lo + gid * per;
+  // each lane walks its own contiguous run of the tile's (nu-sorted) lines,
+  // so the 64 atomics of one instruction land on 64 distant grid points
+  const long long per = (hi - lo + LDM_FILL_T - 1) / LDM_FILL_T;
+  const long long g0 = lo + (long long)threadIdx.x * per;
   const long long g1 = (g0 + per < hi) ? g0 + per : hi;
-  for (long long i = g0 + lane; i < g1; i += LDM_FILL_GROUP) {
+  for (long long i = g0; i < g1; ++i) {
     LineTerms t;
     line_terms(J, db, i, s_laG, s_slG, s_laL, s_slL, ng, nl, t);
-    const long long k0 = t.k0;
-    if (k0 < k_lo - 1 || k0 > k_hi - 1) continue;
-    const int l0 = t.l0, m0 = t.m0;
-    const float iv[2] = {t.I * (1.0f - t.tv), t.I * t.tv};
-    const float ag[2] = {1.0f - t.tg, t.tg};
-    const float al[2] = {1.0f - t.tl, t.tl};
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const long long k = k0 + a;
-      if (k < k_lo || k >= k_hi) continue;
-      const int c = (int)(k - k_lo);
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          const int r = (l0 + b) * nl + (m0 + d);
-          atomicAdd(&acc[r * Wp + c], iv[a] * ag[b] * al[d]);
-        }
+    const int c0 = (int)(t.k0 - k_lo);  // column of corner k0 in the tile, in [-1, W-1] if touching
+    if (c0 < -1 || c0 > (int)(k_hi - k_lo) - 1) continue;
+    // the 8 corners are base + {0, 1} + {0, nl*Wp} + {0, Wp}
+    float* base = acc + (t.l0 * nl + t.m0) * Wp + c0;
+    const int sg = nl * Wp;
+    const float w00 = (1.0f - t.tg) * (1.0f - t.tl), w01 = (1.0f - t.tg) * t.tl;
+    const float w10 = t.tg * (1.0f - t.tl), w11 = t.tg * t.tl;
+    if (c0 >= 0) {
+      const float v = t.I * (1.0f - t.tv);
+      atomicAdd(base, v * w00);
+      atomicAdd(base + Wp, v * w01);
+      atomicAdd(base + sg, v * w10);
+      atomicAdd(base + sg + Wp, v * w11);
+    }
+    if (c0 + 1 < (int)(k_hi - k_lo)) {
+      const float v = t.I * t.tv;
+      atomicAdd(base + 1, v * w00);
+      atomicAdd(base + 1 + Wp, v * w01);
+      atomicAdd(base + 1 + sg, v * w10);
+      atomicAdd(base + 1 + sg + Wp, v * w11);
     }
   }
   __syncthreads();
@@ -459,7 +517,7 @@ __device__ __forceinline__ int xcd_block(int h, int G) {
 // DFTs of columns n2 in [cb*B, cb*B+B) and writes
 //   Y_p[k1][n2] = W_L^(n2 k1) * sum_n1 z[n2 + N2 n1] W_N1^(n1 k1).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(LDM_FFT_T) void k_fft_a(const LdmJob* __restrict__ jobs,
+__global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_a(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
                                                      const int4* __restrict__ items, int n_items) {
   __shared__ float2 buf[LdsSize<LDM_CAP>::value];
@@ -520,7 +578,7 @@ __device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, long long 
 // CAP/T select a small tile (many workgroups, N2 <= 2048) or a large one.
 // ---------------------------------------------------------------------------
 template <int CAP, int T>
-__global__ __launch_bounds__(T) void k_fft_b(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
+__global__ __launch_bounds__(T, 768 / T) void k_fft_b(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
                                              const int4* __restrict__ items) {
   __shared__ float2 buf[LdsSize<CAP>::value];
   const int4 it = items[blockIdx.x];
@@ -535,14 +593,39 @@ __global__ __launch_bounds__(T) void k_fft_b(const LdmJob* __restrict__ jobs, co
   float2 acc[NACC];
 #pragma unroll
   for (int s = 0; s < NACC; ++s) acc[s] = make_float2(0.0f, 0.0f);
+  // element e of a pair's block: row b = e / N2, column n2 = e % N2 (n2
+  // fastest in global memory, transposed into [n2][b] in LDS); the next
+  // pair's block is fetched into registers while this one is transformed
+  float2 pre[NACC];
+#pragma unroll
+  for (int s = 0; s < NACC; ++s) {
+    const int e = threadIdx.x + s * T;
+    if (e < total && P > 0) {
+      const int b = e / N2, n2 = e - b * N2;
+      pre[s] = J.ybuf[(long long)(k1_0 + b) * N2 + n2];
+    }
+  }
   for (int p = 0; p < P; ++p) {
-    const float2* Y = J.ybuf + (long long)p * L;
-    // coalesced: n2 fastest in global memory, transposed into [n2][b] in LDS
-    for (int e = threadIdx.x; e < total; e += T) {
-      const int b = e / N2, n2 = e - (e / N2) * N2;
-      buf[pad_idx((n2 << bs) + b)] = Y[(long long)(k1_0 + b) * N2 + n2];
+#pragma unroll
+    for (int s = 0; s < NACC; ++s) {
+      const int e = threadIdx.x + s * T;
+      if (e < total) {
+        const int b = e / N2, n2 = e - b * N2;
+        buf[pad_idx((n2 << bs) + b)] = pre[s];
+      }
     }
     __syncthreads();
+    if (p + 1 < P) {
+      const float2* Yn = J.ybuf + (long long)(p + 1) * L;
+#pragma unroll
+      for (int s = 0; s < NACC; ++s) {
+        const int e = threadIdx.x + s * T;
+        if (e < total) {
+          const int b = e / N2, n2 = e - b * N2;
+          pre[s] = Yn[(long long)(k1_0 + b) * N2 + n2];
+        }
+      }
+    }
     lds_fft<true, CAP, T>(buf, N2, bs, J.rad2, J.nrad2, J.tw2);
     const int ra = 2 * p, rb = 2 * p + 1;
 #pragma unroll
@@ -580,7 +663,7 @@ __global__ __launch_bounds__(T) void k_fft_b(const LdmJob* __restrict__ jobs, co
 // sum_k1 B[k1][n2] W_N1^(-n1 k1); abscoeff[n] = Re(y[n]) / (L wstep) for
 // n < N_v (radis/lbl/broadening.py:2529-2530 crop and /wstep).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(LDM_FFT_T) void k_fft_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items,
+__global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                                      int n_items) {
   __shared__ float2 buf[LdsSize<LDM_CAP>::value];
   const int4 it = items[xcd_block(blockIdx.x, n_items)];

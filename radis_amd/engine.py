@@ -181,7 +181,7 @@ class LineDatabase:
 
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
-                 fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False):
+                 fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
@@ -196,6 +196,11 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
     r.flags = _native.FP64_WIDTHS if fp64_widths else 0
+    if not (fft_n1 or tile_w):
+        from . import autotune
+
+        fft_n1, tile_w = autotune.lookup(r)
+    r.fft_n1, r.tile_w = int(fft_n1 or 0), int(tile_w or 0)
     if line_window is not None:
         r.line_lo, r.line_hi = float(line_window[0]), float(line_window[1])
     return r
