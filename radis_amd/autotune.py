@@ -16,7 +16,7 @@ import os
 
 _CACHE = None
 _MAX_N1, _MAX_N2 = 1024, 8192
-_FILL_LDS = 56 * 1024
+_FILL_LDS = 60 * 1024
 
 
 def cache_path():
@@ -110,7 +110,7 @@ def tune(db, req, reps=5, tile_widths=(64, 128, 256, 512, 1024), verbose=False):
     nv = _grid_len(req)
     _, info = db.request_bytes(req)
     m_cap = int(info.m_cap)
-    widths = [w for w in tile_widths if m_cap * (w + 1) * 4 <= _FILL_LDS] or [0]
+    widths = [w for w in tile_widths if m_cap * (w + 2) * 8 <= _FILL_LDS] or [0]
     results = []
     for n1 in fft_candidates(nv) or [0]:
         for w in widths:

@@ -30,7 +30,8 @@ const double HC_K = K_H * K_C / K_KB * 100;  // constants.py:69
 const double ATM_PER_BAR = 1.01325;          // lbl/broadening.py:1063
 
 const int STAT_BLOCK = 4096;             // lines per host-side statistics block
-const int FILL_LDS_TARGET = 56 * 1024;   // bytes of LDS for one fill tile (plus 2 KB of axis tables)
+const int FILL_LDS_TARGET = 24 * 1024;   // preferred LDS bytes of one fill tile (FP64 cells; ~4 tiles per CU)
+const int FILL_LDS_MAX = 60 * 1024;      // hard limit (plus ~2.2 KB of static axis tables, < 64 KB)
 const int MIN_TILE_W = 16, MAX_TILE_W = 4096;
 const int N1_MAX = 1024, N2_MAX = 8192;
 
@@ -397,9 +398,11 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   return 0;
 }
 
+size_t fill_tile_bytes(int m_cap, int w) { return (size_t)m_cap * (size_t)(w + LDM_FILL_PAD) * sizeof(double); }
+
 int choose_tile(int m_cap) {
   int w = MAX_TILE_W;
-  while (w > MIN_TILE_W && (size_t)m_cap * (w + 1) * 4 > (size_t)FILL_LDS_TARGET) w >>= 1;
+  while (w > MIN_TILE_W && fill_tile_bytes(m_cap, w) > (size_t)FILL_LDS_TARGET) w >>= 1;
   return w;
 }
 
@@ -494,13 +497,13 @@ int build_layout(radis_ldm_batch* b) {
     if (R.plan.n2 > LDM_CAP_B) b->small_b = false;
   for (Req& R : b->reqs) {
     R.tile_w = R.r.tile_w ? R.r.tile_w : choose_tile(R.m_cap);
-    if ((size_t)R.m_cap * (R.tile_w + 1) * 4 > (size_t)FILL_LDS_TARGET)
-      return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit the fill tile (max %d rows)", R.m_cap,
-                  FILL_LDS_TARGET / (4 * MIN_TILE_W));
+    if (fill_tile_bytes(R.m_cap, R.tile_w) > (size_t)FILL_LDS_MAX)
+      return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit a fill tile of width %d (max %d rows at width %d)",
+                  R.m_cap, R.tile_w, (int)(FILL_LDS_MAX / fill_tile_bytes(1, MIN_TILE_W)), MIN_TILE_W);
     const int64_t V = R.nv + 1;
     R.n_tiles = (int)((V + R.tile_w - 1) / R.tile_w);
     R.stride = (int64_t)align_up((size_t)V, 64);
-    fill_lds = std::max(fill_lds, (size_t)R.m_cap * (R.tile_w + 1) * 4);
+    fill_lds = std::max(fill_lds, fill_tile_bytes(R.m_cap, R.tile_w));
     const int n1 = R.plan.n1, n2 = R.plan.n2;
     R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
     R.bB = std::min(floor_log2((b->small_b ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
@@ -834,11 +837,11 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
     db->s_max[b] = std::max(db->s_max[b], L->S_ref[i]);
   }
   // device arrays: one allocation
-  size_t off[9], tot = 0;
-  const size_t sz[9] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
-                        db->has_nself ? (size_t)n * 4 : 0, db->has_delta ? (size_t)n * 4 : 0,
-                        db->has_iso ? (size_t)n : 0};
-  for (int k = 0; k < 9; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
+  size_t off[10], tot = 0;
+  const size_t sz[10] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
+                         db->has_nself ? (size_t)n * 4 : 0, db->has_delta ? (size_t)n * 4 : 0,
+                         db->has_iso ? (size_t)n : 0, 8};
+  for (int k = 0; k < 10; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
   if (ctx->budget > 0 && ctx->in_use + (int64_t)tot > ctx->budget) {
     delete db;
     return fail(RADIS_LDM_ENOMEM, "line list of %zu bytes exceeds the device budget", tot);
@@ -852,8 +855,10 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   ctx->refs++;
   ctx->in_use += (int64_t)tot;
   char* m = (char*)db->mem;
-  const void* src[9] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso};
-  for (int k = 0; k < 9; ++k)
+  // defaults block: a float 0 (missing delta column) and a byte 1 (missing iso column)
+  unsigned char dflt[8] = {0, 0, 0, 0, 1, 0, 0, 0};
+  const void* src[10] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso, dflt};
+  for (int k = 0; k < 10; ++k)
     if (sz[k]) {
       hipError_t e = hipMemcpy(m + off[k], src[k], sz[k], hipMemcpyHostToDevice);
       if (e != hipSuccess) {
@@ -870,6 +875,7 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   db->dev.n_self = db->has_nself ? (const float*)(m + off[6]) : nullptr;
   db->dev.delta = db->has_delta ? (const float*)(m + off[7]) : nullptr;
   db->dev.iso = db->has_iso ? (const uint8_t*)(m + off[8]) : nullptr;
+  db->dev.dflt = (const float*)(m + off[9]);
   *out = db;
   return 0;
 }

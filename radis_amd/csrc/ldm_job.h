@@ -18,6 +18,7 @@
 #define LDM_CAP_B 2048     // complex elements held by a small pass B workgroup
 #define LDM_FFT_T_B 256    // threads of a small pass B workgroup
 #define LDM_FILL_T 256     // threads of a fill workgroup
+#define LDM_FILL_PAD 2     // LDS row stride of a fill tile: tile_w + LDM_FILL_PAD doubles (16-byte aligned rows)
 #define LDM_AX_CHUNK 4096  // lines per axis-reduction workgroup (16 per lane)
 #define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
 #define LDM_NFOLD_MAX 100000
@@ -39,6 +40,7 @@ struct LdmDb {
   const float* n_self;   // may be null
   const float* delta;    // may be null
   const uint8_t* iso;    // may be null
+  const float* dflt;     // device block {0.0f, then byte 1}: what a missing delta / iso column reads
 };
 
 struct LdmJob {

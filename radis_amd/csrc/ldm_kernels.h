@@ -31,9 +31,10 @@
 //   x < w[0] -> 0; x >= w[nv-1] -> nv-1; w[j] == x -> j;
 //   else (1/(w[j+1]-w[j]))*(x-w[j]) + j with w[j] <= x < w[j+1].
 // radis/lbl/broadening.py:2151-2154, 2253.
-__device__ __forceinline__ double grid_position(double x, double wmin, double delta, double inv_delta,
-                                                long long nv) {
+__device__ __forceinline__ double grid_position(double x, double wmin, double delta, double inv_delta, int nv) {
 #pragma clang fp contract(off)
+  // grids are shorter than 2^31 points (the FFT length caps them far below),
+  // so 32-bit indices convert to and from FP64 in one instruction
   const double wlast = wmin + (double)(nv - 1) * delta;
   if (x < wmin) return 0.0;
   if (!(x < wlast)) return (double)(nv - 1);
@@ -41,8 +42,8 @@ __device__ __forceinline__ double grid_position(double x, double wmin, double de
   // ~1e-9 grid steps for grids below 1e7 points, so the largest j with
   // w[j] <= x is the guess or one of its neighbours; one exact comparison
   // each way settles it.
-  long long j = (long long)floor((x - wmin) * inv_delta);
-  j = j < 0 ? 0 : (j > nv - 2 ? nv - 2 : j);
+  int j = (int)((x - wmin) * inv_delta);  // x > wmin: truncation is floor
+  j = j > nv - 2 ? nv - 2 : j;
   double wj = wmin + (double)j * delta;
   if (wj > x && j > 0) {
     --j;
@@ -366,39 +367,133 @@ __device__ __forceinline__ float boltzmann_scale(double s, double x) {
 }
 
 struct LineTerms {
-  long long k0;
+  int k0;
   int l0, m0;
   float tv, tg, tl, I;
 };
+
+// One line as stored in the device columns.
+struct LineRaw {
+  double nu0;
+  float dl, ga, gs, na, ns, S, El;
+  int iso;
+};
+
+// Column pointers with the optional columns made unconditional: a missing
+// delta column reads 0.0f and a missing iso column reads 1 (both at index 0
+// of the database's defaults block, selected by an index mask of 0), a
+// missing n_self column reads n_air.  Every load is then valid for any line
+// index, so loads of several lines can be in flight with no branch between.
+struct LineCols {
+  const double* nu0;
+  const float *S, *El, *ga, *gs, *na, *ns, *dl;
+  const uint8_t* iso;
+  long long dmask, imask;
+  __device__ explicit LineCols(const LdmDb& db)
+      : nu0(db.nu0), S(db.S_ref), El(db.El), ga(db.g_air), gs(db.g_self), na(db.n_air),
+        ns(db.n_self ? db.n_self : db.n_air), dl(db.delta ? db.delta : db.dflt),
+        iso(db.iso ? db.iso : reinterpret_cast<const uint8_t*>(db.dflt + 1)),
+        dmask(db.delta ? -1LL : 0LL), imask(db.iso ? -1LL : 0LL) {}
+  __device__ __forceinline__ void load(long long i, LineRaw& r) const {
+    r.iso = iso[i & imask];  // first: the per-isotopologue lookups wait on it alone
+    r.nu0 = nu0[i];
+    r.dl = dl[i & dmask];
+    r.ga = ga[i];
+    r.gs = gs[i];
+    r.na = na[i];
+    r.ns = ns[i];
+    r.S = S[i];
+    r.El = El[i];
+  }
+};
+
+// Per-isotopologue values of a line, read from the job table in global
+// memory: vector loads, whose completion counter is independent of the LDS
+// atomics (an LDS table read would make every line wait for the previous
+// line's atomics to retire).
+struct IsoVals {
+  float lgc;   // log(2*dopp_iso/c_CGS)
+  double q;    // Q(Tref)/Q(T)
+};
+
+__device__ __forceinline__ void iso_vals(const LdmJob& J, int iso, IsoVals& v) {
+  v.lgc = J.lgc_f[iso];
+  v.q = J.qratio[iso];
+}
+
+// Request constants of the per-line computation, read once per workgroup;
+// the per-isotopologue values live in LDS (indexed per lane).
+struct LineConst {
+  double wmin, delta_w, inv_delta_w;
+  double p_atm, x_air, x_self, log_r;
+  double c_boltz, i_scale;
+  float c_stim_T, c_stim_R, log_r_f, pa_air_f, pa_self_f;
+  float laG0_f, inv_dxG_f, laL0_f, inv_dxL_f;
+  int nv, fp64_widths, has_delta;
+  __device__ explicit LineConst(const LdmJob& J, const LdmDb& db)
+      : wmin(J.wmin), delta_w(J.delta_w), inv_delta_w(J.inv_delta_w), p_atm(J.p_atm), x_air(J.x_air),
+        x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
+        c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
+        laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
+        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr) {}
+};
+
+// Per-workgroup LDS tables of the line computation.
+struct LineTables {
+  double laG[LDM_MAX_AX], slG[LDM_MAX_AX], laL[LDM_MAX_AX], slL[LDM_MAX_AX];
+  double dopp[LDM_MAX_ISO];
+};
+
+__device__ __forceinline__ void load_tables(const LdmJob& J, const LdmAxes& A, LineTables& T) {
+  for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
+    T.laG[e] = A.laG[e]; T.slG[e] = A.slG[e];
+    T.laL[e] = A.laL[e]; T.slL[e] = A.slL[e];
+  }
+  for (int e = threadIdx.x; e < LDM_MAX_ISO; e += blockDim.x) {
+    T.dopp[e] = J.dopp[e];
+  }
+}
 
 __device__ __forceinline__ bool near_node(float p, int n) {
   const float f = p - floorf(p);
   return f < LDM_TIE_EPS || f > 1.0f - LDM_TIE_EPS || p < LDM_TIE_EPS || p > (float)(n - 1) - LDM_TIE_EPS;
 }
 
-__device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, long long i, const double* laG,
-                                           const double* slG, const double* laL, const double* slL, int ng, int nl,
-                                           LineTerms& t) {
-  const double nu0 = db.nu0[i];
-  double nu = nu0;
-  if (db.delta) {
+// FP64 Lorentzian FWHM from the request constants (same expression as
+// fwhm_lorentz).
+__device__ __forceinline__ double fwhm_lorentz_c(const LineConst& C, double g_air, double g_self, double n_air,
+                                                 double n_self) {
 #pragma clang fp contract(off)
-    nu = nu0 + ((double)db.delta[i] * J.p_atm);
+  const double ra = exp(n_air * C.log_r);
+  const double rs = (n_self == n_air) ? ra : exp(n_self * C.log_r);
+  const double air = ((ra * g_air) * C.p_atm) * C.x_air;
+  const double slf = rs * ((g_self * C.p_atm) * C.x_self);
+  return (air + slf) * 2.0;
+}
+
+__device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r, const IsoVals& iv,
+                                           const LineTables& T, int ng, int nl, LineTerms& t) {
+  const double nu0 = r.nu0;
+  double nu = nu0;
+  if (C.has_delta) {
+#pragma clang fp contract(off)
+    nu = nu0 + ((double)r.dl * C.p_atm);
   }
-  const double pos = grid_position(nu, J.wmin, J.delta_w, J.inv_delta_w, J.nv);
-  t.k0 = (long long)pos;
+  const double pos = grid_position(nu, C.wmin, C.delta_w, C.inv_delta_w, C.nv);
+  t.k0 = (int)pos;
   t.tv = (float)(pos - (double)t.k0);
-  const int iso = db.iso ? db.iso[i] : 1;
-  const float ga = db.g_air[i], gs = db.g_self[i], na = db.n_air[i];
-  const float ns = db.n_self ? db.n_self[i] : na;
+  const int iso = r.iso;
   const float nuf = (float)nu0;
   // FP32 positions on the log axes
-  float pg = (__logf(nuf) + J.lgc_f[iso] - J.laG0_f) * J.inv_dxG_f;
-  float pl = (__logf(fwhm_lorentz_f32(J, ga, gs, na, ns)) - J.laL0_f) * J.inv_dxL_f;
-  const bool exact_g = J.fp64_widths || near_node(pg, ng);
-  const bool exact_l = J.fp64_widths || near_node(pl, nl);
+  const float ra = __expf(r.na * C.log_r_f);
+  const float rs = (r.ns == r.na) ? ra : __expf(r.ns * C.log_r_f);
+  const float wl_f = 2.0f * (ra * r.ga * C.pa_air_f + rs * r.gs * C.pa_self_f);  // as fwhm_lorentz_f32
+  const float pg = (__logf(nuf) + iv.lgc - C.laG0_f) * C.inv_dxG_f;
+  const float pl = (__logf(wl_f) - C.laL0_f) * C.inv_dxL_f;
+  const bool exact_g = C.fp64_widths || near_node(pg, ng);
+  const bool exact_l = C.fp64_widths || near_node(pl, nl);
   if (exact_g) {
-    const double p = axis_position(log(fwhm_gauss(nu0, J.dopp[iso])), laG, slG, ng);
+    const double p = axis_position(log(fwhm_gauss(nu0, T.dopp[iso])), T.laG, T.slG, ng);
     t.l0 = (int)p;
     t.tg = (float)(p - (double)t.l0);
   } else {
@@ -407,7 +502,7 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
     t.tg = pg - f;
   }
   if (exact_l) {
-    const double p = axis_position(log(fwhm_lorentz(J, ga, gs, na, ns)), laL, slL, nl);
+    const double p = axis_position(log(fwhm_lorentz_c(C, r.ga, r.gs, r.na, r.ns)), T.laL, T.slL, nl);
     t.m0 = (int)p;
     t.tl = (float)(p - (double)t.m0);
   } else {
@@ -417,9 +512,9 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
   }
   if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
   if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
-  const double s64 = (double)db.S_ref[i] * J.qratio[iso] * J.i_scale;
-  const float stim = expm1f(J.c_stim_T * nuf) / expm1f(J.c_stim_R * nuf);
-  t.I = boltzmann_scale(s64, J.c_boltz * (double)db.El[i]) * stim;
+  const double s64 = (double)r.S * iv.q * C.i_scale;
+  const float stim = expm1f(C.c_stim_T * nuf) / expm1f(C.c_stim_R * nuf);
+  t.I = boltzmann_scale(s64, C.c_boltz * (double)r.El) * stim;
 }
 
 // ---------------------------------------------------------------------------
@@ -428,69 +523,110 @@ __device__ __forceinline__ void line_terms(const LdmJob& J, const LdmDb& db, lon
 // simple weights (radis/lbl/broadening.py:2305-2318) of its corners that
 // fall in the tile into LDS with ds_add_f32.  The tile is written to HBM
 // once, so the LDM needs no clearing and no global atomics.  Row index
-// r = l*N_L + m; LDS row stride W+1 so rows fall on different banks.
+// r = l*N_L + m; LDS row stride W+2 doubles (16-byte aligned rows, rows on
+// different banks).  The tile accumulates in FP64 with ds_add_f64: on gfx950
+// LDS ds_add_f32 retires far slower than the 64-bit and integer LDS atomics
+// (measured ~3 cycles per lane per CU, which bounded this kernel), and the
+// FP64 sums are also more accurate.
 // Each lane walks its own contiguous run of the tile's nu-sorted lines: the
 // 64 lanes of an atomic instruction then target 64 distant grid points, so
-// same-address serialisation in LDS is rare.
+// same-address serialisation in LDS is rare.  The raw columns of the next
+// two lines are loaded while the current one is computed, and no LDS read
+// sits on the common path (only atomics), so atomics retire in the background.
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
                                                      const int4* __restrict__ items, LdmDb db) {
-  extern __shared__ float acc[];
-  __shared__ double s_laG[LDM_MAX_AX], s_slG[LDM_MAX_AX], s_laL[LDM_MAX_AX], s_slL[LDM_MAX_AX];
+  extern __shared__ __align__(16) double acc[];
+  __shared__ LineTables tab;
   const int4 it = items[blockIdx.x];
   const LdmJob& J = jobs[it.x];
   const LdmAxes& A = axes[it.x];
   const int M = A.m;
   if (M == 0 || J.status) return;
   const int ng = A.ng, nl = A.nl;
-  const int W = J.tile_w, Wp = W + 1;
+  const int W = J.tile_w, Wp = W + LDM_FILL_PAD;
   const long long k_lo = (long long)it.y * W;
   long long k_hi = k_lo + W;
   if (k_hi > J.v) k_hi = J.v;
-  for (int e = threadIdx.x; e < M * Wp; e += blockDim.x) acc[e] = 0.0f;
-  for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
-    s_laG[e] = A.laG[e]; s_slG[e] = A.slG[e];
-    s_laL[e] = A.laL[e]; s_slL[e] = A.slL[e];
-  }
-  __syncthreads();
+  const LineConst C(J, db);
+  const LineCols cols(db);
   const long long lo = J.tiles[2 * it.y], hi = J.tiles[2 * it.y + 1];
   // each lane walks its own contiguous run of the tile's (nu-sorted) lines,
   // so the 64 atomics of one instruction land on 64 distant grid points
   const long long per = (hi - lo + LDM_FILL_T - 1) / LDM_FILL_T;
   const long long g0 = lo + (long long)threadIdx.x * per;
   const long long g1 = (g0 + per < hi) ? g0 + per : hi;
+  // first two lines in flight before the LDS set-up (indices clamped into
+  // [g0, g1 - 1], which is a valid line whenever the loop runs)
+  const long long glast = g1 - 1 > g0 ? g1 - 1 : g0;
+  LineRaw n1, n2;
+  IsoVals v1;
+  if (g0 < g1) {
+    cols.load(g0, n1);
+    iso_vals(J, n1.iso, v1);
+    cols.load(g0 + 1 < glast ? g0 + 1 : glast, n2);
+  }
+  {
+    const int n2 = (M * Wp) >> 1;  // Wp is even: whole 16-byte pairs
+    double2* a2 = reinterpret_cast<double2*>(acc);
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) a2[e] = make_double2(0.0, 0.0);
+  }
+  load_tables(J, A, tab);
+  __syncthreads();
+  const int wlen = (int)(k_hi - k_lo);
+  const int sg = nl * Wp;
   for (long long i = g0; i < g1; ++i) {
+    const LineRaw cur = n1;
+    const IsoVals cv = v1;
+    n1 = n2;
+    iso_vals(J, n1.iso, v1);
+    cols.load(i + 2 < glast ? i + 2 : glast, n2);
     LineTerms t;
-    line_terms(J, db, i, s_laG, s_slG, s_laL, s_slL, ng, nl, t);
-    const int c0 = (int)(t.k0 - k_lo);  // column of corner k0 in the tile, in [-1, W-1] if touching
-    if (c0 < -1 || c0 > (int)(k_hi - k_lo) - 1) continue;
+    line_terms(C, cur, cv, tab, ng, nl, t);
+    const int c0 = t.k0 - (int)k_lo;  // column of corner k0 in the tile, in [-1, W-1] if touching
+    if (c0 < -1 || c0 > wlen - 1) continue;
     // the 8 corners are base + {0, 1} + {0, nl*Wp} + {0, Wp}
-    float* base = acc + (t.l0 * nl + t.m0) * Wp + c0;
-    const int sg = nl * Wp;
+    double* base = acc + (t.l0 * nl + t.m0) * Wp + c0;
     const float w00 = (1.0f - t.tg) * (1.0f - t.tl), w01 = (1.0f - t.tg) * t.tl;
     const float w10 = t.tg * (1.0f - t.tl), w11 = t.tg * t.tl;
     if (c0 >= 0) {
       const float v = t.I * (1.0f - t.tv);
-      atomicAdd(base, v * w00);
-      atomicAdd(base + Wp, v * w01);
-      atomicAdd(base + sg, v * w10);
-      atomicAdd(base + sg + Wp, v * w11);
+      atomicAdd(base, (double)(v * w00));
+      atomicAdd(base + Wp, (double)(v * w01));
+      atomicAdd(base + sg, (double)(v * w10));
+      atomicAdd(base + sg + Wp, (double)(v * w11));
     }
-    if (c0 + 1 < (int)(k_hi - k_lo)) {
+    if (c0 + 1 < wlen) {
       const float v = t.I * t.tv;
-      atomicAdd(base + 1, v * w00);
-      atomicAdd(base + 1 + Wp, v * w01);
-      atomicAdd(base + 1 + sg, v * w10);
-      atomicAdd(base + 1 + sg + Wp, v * w11);
+      atomicAdd(base + 1, (double)(v * w00));
+      atomicAdd(base + 1 + Wp, (double)(v * w01));
+      atomicAdd(base + 1 + sg, (double)(v * w10));
+      atomicAdd(base + 1 + sg + Wp, (double)(v * w11));
     }
   }
   __syncthreads();
-  const int wlen = (int)(k_hi - k_lo);
-  for (int e = threadIdx.x; e < M * W; e += blockDim.x) {
-    const int r = e / W, c = e - (e / W) * W;
-    if (c < wlen) J.ldm[(long long)r * J.ldm_stride + k_lo + c] = acc[r * Wp + c];
+  // flush: W is a power of two and LDS rows are 16-byte aligned; a lane owns
+  // 4 columns and walks the rows (W/4 >= T: one row at a time over several
+  // column groups; W/4 < T: T/(W/4) rows at once)
+  const int W4 = W >> 2, sh = __builtin_ctz(W4);
+  const bool wide = W4 >= LDM_FILL_T;
+  const int r0 = wide ? 0 : (int)(threadIdx.x >> sh);
+  const int rstep = wide ? 1 : (LDM_FILL_T >> sh);
+  const long long dstep = (long long)rstep * J.ldm_stride;
+  for (int c = 4 * (int)(threadIdx.x & (W4 - 1)); c < wlen; c += 4 * LDM_FILL_T) {
+    float* dst = J.ldm + (k_lo + c) + (long long)r0 * J.ldm_stride;
+    const double* src = acc + r0 * Wp + c;
+    if (c + 4 <= wlen) {
+      for (int r = r0; r < M; r += rstep, dst += dstep, src += rstep * Wp) {
+        const double2 a = reinterpret_cast<const double2*>(src)[0], b = reinterpret_cast<const double2*>(src)[1];
+        *reinterpret_cast<float4*>(dst) = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+      }
+    } else {
+      for (int r = r0; r < M; r += rstep, dst += dstep, src += rstep * Wp)
+        for (int q = 0; q < wlen - c; ++q) dst[q] = (float)src[q];
+    }
   }
 }
 
@@ -699,19 +835,22 @@ __global__ __launch_bounds__(256) void k_line_terms(const LdmJob* __restrict__ j
                                                     int job, LdmDb db, long long* __restrict__ k0, int* __restrict__ l0,
                                                     int* __restrict__ m0, float* __restrict__ tv, float* __restrict__ tg,
                                                     float* __restrict__ tl, float* __restrict__ I) {
-  __shared__ double s_laG[LDM_MAX_AX], s_slG[LDM_MAX_AX], s_laL[LDM_MAX_AX], s_slL[LDM_MAX_AX];
+  __shared__ LineTables tab;
   const LdmJob& J = jobs[job];
   const LdmAxes& A = axes[job];
-  for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
-    s_laG[e] = A.laG[e]; s_slG[e] = A.slG[e];
-    s_laL[e] = A.laL[e]; s_slL[e] = A.slL[e];
-  }
+  load_tables(J, A, tab);
   __syncthreads();
   if (A.m == 0 || J.status) return;
+  const LineConst C(J, db);
+  const LineCols cols(db);
   const long long n = J.i1 - J.i0;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    LineRaw r;
+    cols.load(J.i0 + q, r);
+    IsoVals iv;
+    iso_vals(J, r.iso, iv);
     LineTerms t;
-    line_terms(J, db, J.i0 + q, s_laG, s_slG, s_laL, s_slL, A.ng, A.nl, t);
+    line_terms(C, r, iv, tab, A.ng, A.nl, t);
     k0[q] = t.k0; l0[q] = t.l0; m0[q] = t.m0;
     tv[q] = t.tv; tg[q] = t.tg; tl[q] = t.tl; I[q] = t.I;
   }
