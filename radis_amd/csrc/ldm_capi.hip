@@ -17,6 +17,7 @@
 #include "radis_ldm.h"
 #include "ldm_job.h"
 #include "ldm_kernels.h"
+#include "ldm_fftc.h"
 
 namespace {
 
@@ -170,8 +171,16 @@ void db_release(radis_ldm_db* db) {
 struct FftPlan {
   int64_t L = 0;
   int n1 = 0, n2 = 0;
+  bool fc = false;  // compile-time-sized passes (ldm_fftc.h) for this n1, n2
   std::vector<int> r1, r2;
 };
+
+// Sizes with compile-time-sized passes: n1 (passes A and C) and n2 (pass B).
+const int FC_N1[] = {256, 320, 384, 512, 640, 768, 1024};
+const int FC_N2[] = {256, 512, 1024, 2048};
+// A compiled split is preferred while its length is within this factor of
+// the smallest generic (7-smooth) length: its passes run ~3x faster.
+const double FC_SLACK = 1.35;
 
 struct Req {
   radis_ldm_request r{};
@@ -210,7 +219,11 @@ struct radis_ldm_batch {
   int4* items_d = nullptr;
   size_t items_bytes = 0;
   int n_axis = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
-  bool small_b = true;  // pass B with LDM_CAP_B tiles (every n2 <= LDM_CAP_B)
+  // launches of the FFT passes: items of one kernel variant are contiguous;
+  // cls = the compiled size (n1 for A and C, n2 for B), 0 = generic (pass B:
+  // 0 = generic with small tiles, -1 = generic with large tiles)
+  struct Group { int cls, first, count; };
+  std::vector<Group> grp_a, grp_b, grp_c;
   bool tiles_valid = false;  // fill tile bounds computed for the current layout
   size_t fill_lds = 0;
   std::vector<float*> outs;        // outputs of the last eager run
@@ -305,6 +318,26 @@ int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0) {
   out.L = use_small ? small : any;
   out.n1 = use_small ? s1 : a1;
   out.n2 = use_small ? s2 : a2;
+  out.fc = false;
+  // compiled split: smallest L = n1*n2 >= need (or == want), then n1 closest to 512
+  int64_t bestc = -1;
+  int c1 = 0, c2 = 0;
+  for (int n1 : FC_N1) {
+    if (only_n1 && n1 != only_n1) continue;
+    for (int n2 : FC_N2) {
+      const int64_t L = (int64_t)n1 * n2;
+      if (want > 0 ? L != want : L < need) continue;
+      if (bestc < 0 || L < bestc || (L == bestc && std::abs(n1 - 512) < std::abs(c1 - 512))) {
+        bestc = L; c1 = n1; c2 = n2;
+      }
+    }
+  }
+  if (bestc > 0 && (double)bestc <= FC_SLACK * (double)out.L) {
+    out.L = bestc;
+    out.n1 = c1;
+    out.n2 = c2;
+    out.fc = true;
+  }
   if (!radices_of(out.n1, out.r1) || !radices_of(out.n2, out.r2))
     return fail(RADIS_LDM_ERANGE, "radix factorisation failed");
   return 0;
@@ -492,9 +525,6 @@ int build_layout(radis_ldm_batch* b) {
   // --- workspace
   size_t total = 0;
   size_t fill_lds = 0;
-  b->small_b = true;
-  for (const Req& R : b->reqs)
-    if (R.plan.n2 > LDM_CAP_B) b->small_b = false;
   for (Req& R : b->reqs) {
     R.tile_w = R.r.tile_w ? R.r.tile_w : choose_tile(R.m_cap);
     if (fill_tile_bytes(R.m_cap, R.tile_w) > (size_t)FILL_LDS_MAX)
@@ -505,8 +535,13 @@ int build_layout(radis_ldm_batch* b) {
     R.stride = (int64_t)align_up((size_t)V, 64);
     fill_lds = std::max(fill_lds, fill_tile_bytes(R.m_cap, R.tile_w));
     const int n1 = R.plan.n1, n2 = R.plan.n2;
-    R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
-    R.bB = std::min(floor_log2((b->small_b ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
+    if (R.plan.fc) {
+      R.bA = floor_log2(fc_cols_a(n1));
+      R.bB = floor_log2(fc_rows_b(n2));
+    } else {
+      R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
+      R.bB = std::min(floor_log2((n2 <= LDM_CAP_B ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
+    }
     const int pcap = (R.m_cap + 1) / 2;
     R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
     R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
@@ -571,8 +606,9 @@ int build_layout(radis_ldm_batch* b) {
     R.twl_hi = (const float2*)(base + l_off[R.plan.L]);
     R.twl_lo = R.twl_hi + (size_t)((R.plan.L + 1023) >> LDM_TW_LO_BITS);
   }
-  // --- work items
-  std::vector<int4> ax, fi, fa, fb, fc;
+  // --- work items (FFT passes grouped by kernel variant)
+  std::vector<int4> ax, fi;
+  std::map<int, std::vector<int4>> ma, mb, mc;
   for (int j = 0; j < b->n; ++j) {
     const Req& R = b->reqs[j];
     const int64_t nl = R.i1 - R.i0;
@@ -580,11 +616,25 @@ int build_layout(radis_ldm_batch* b) {
     for (int t = 0; t < R.n_tiles; ++t) fi.push_back(make_int4(j, t, 0, 0));
     const int pcap = (R.m_cap + 1) / 2;
     const int nA = R.plan.n2 >> R.bA;
+    const int ca_ = R.plan.fc ? R.plan.n1 : 0;
+    const int cb_ = R.plan.fc ? R.plan.n2 : (R.plan.n2 <= LDM_CAP_B ? 0 : -1);
     for (int p = 0; p < pcap; ++p)
-      for (int cb = 0; cb < nA; ++cb) fa.push_back(make_int4(j, p, cb, 0));
-    for (int kb = 0; kb < (R.plan.n1 >> R.bB); ++kb) fb.push_back(make_int4(j, kb, 0, 0));
-    for (int cb = 0; cb < nA; ++cb) fc.push_back(make_int4(j, 0, cb, 0));
+      for (int cb = 0; cb < nA; ++cb) ma[ca_].push_back(make_int4(j, p, cb, 0));
+    for (int kb = 0; kb < (R.plan.n1 >> R.bB); ++kb) mb[cb_].push_back(make_int4(j, kb, 0, 0));
+    for (int cb = 0; cb < nA; ++cb) mc[ca_].push_back(make_int4(j, 0, cb, 0));
   }
+  std::vector<int4> fa, fb, fc;
+  auto flatten = [](std::map<int, std::vector<int4>>& m, std::vector<int4>& v,
+                    std::vector<radis_ldm_batch::Group>& g) {
+    g.clear();
+    for (auto& kv : m) {
+      g.push_back({kv.first, (int)v.size(), (int)kv.second.size()});
+      v.insert(v.end(), kv.second.begin(), kv.second.end());
+    }
+  };
+  flatten(ma, fa, b->grp_a);
+  flatten(mb, fb, b->grp_b);
+  flatten(mc, fc, b->grp_c);
   b->n_axis = (int)ax.size();
   b->n_fill = (int)fi.size();
   b->n_a = (int)fa.size();
@@ -640,16 +690,24 @@ int download_results(radis_ldm_batch* b, hipStream_t s, int slot) {
 
 // Launch one kernel; in a timed run with per-dispatch start/stop events,
 // which the runtime stamps on the kernel's own packet.
+// One launch of a stage made of several launches: the stage's start event
+// is recorded by its first launch and its stop event by its last.
 template <typename K, typename... Args>
-void launch(radis_ldm_batch* b, bool timed, int slot, int kid, K kernel, dim3 grid, dim3 block, size_t shmem,
-            hipStream_t s, Args... args) {
+void launch_part(radis_ldm_batch* b, bool timed, int slot, int kid, bool first, bool last, K kernel, dim3 grid,
+                 dim3 block, size_t shmem, hipStream_t s, Args... args) {
   if (timed) {
-    hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, s, b->st_ev[slot][2 * kid],
-                          b->st_ev[slot][2 * kid + 1], 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, s, first ? b->st_ev[slot][2 * kid] : nullptr,
+                          last ? b->st_ev[slot][2 * kid + 1] : nullptr, 0, args...);
     b->ev_used[slot][kid] = true;
   } else {
     hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
   }
+}
+
+template <typename K, typename... Args>
+void launch(radis_ldm_batch* b, bool timed, int slot, int kid, K kernel, dim3 grid, dim3 block, size_t shmem,
+            hipStream_t s, Args... args) {
+  launch_part(b, timed, slot, kid, true, true, kernel, grid, block, shmem, s, args...);
 }
 
 // stage of each kernel id: axis, finalize, tile bounds (fill), fill, fft_a, fft_b, fft_c
@@ -677,17 +735,53 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
     launch(b, timed, slot, 3, k_fill, dim3(b->n_fill), dim3(LDM_FILL_T), b->fill_lds, s, cj, ca, items, db.dev);
   }
   items += b->n_fill;
-  if (b->n_a) launch(b, timed, slot, 4, k_fft_a, dim3(b->n_a), dim3(LDM_FFT_T), 0, s, cj, ca, items, b->n_a);
+  const int ng_a = (int)b->grp_a.size(), ng_b = (int)b->grp_b.size(), ng_c = (int)b->grp_c.size();
+  for (int g = 0; g < ng_a; ++g) {
+    const auto& G = b->grp_a[g];
+    const int4* it = items + G.first;
+    const bool f = g == 0, l = g == ng_a - 1;
+    const dim3 grid(G.count);
+#define FC_A(N) \
+  case N: launch_part(b, timed, slot, 4, f, l, k_fft_a_c<N>, grid, dim3(FC_T), 0, s, cj, ca, it, G.count); break;
+    switch (G.cls) {
+      FC_A(256) FC_A(320) FC_A(384) FC_A(512) FC_A(640) FC_A(768) FC_A(1024)
+      default: launch_part(b, timed, slot, 4, f, l, k_fft_a, grid, dim3(LDM_FFT_T), 0, s, cj, ca, it, G.count);
+    }
+#undef FC_A
+  }
   items += b->n_a;
-  if (b->n_b) {
-    if (b->small_b)
-      launch(b, timed, slot, 5, k_fft_b<LDM_CAP_B, LDM_FFT_T_B>, dim3(b->n_b), dim3(LDM_FFT_T_B), 0, s, cj, ca,
-             items);
-    else
-      launch(b, timed, slot, 5, k_fft_b<LDM_CAP, LDM_FFT_T>, dim3(b->n_b), dim3(LDM_FFT_T), 0, s, cj, ca, items);
+  for (int g = 0; g < ng_b; ++g) {
+    const auto& G = b->grp_b[g];
+    const int4* it = items + G.first;
+    const bool f = g == 0, l = g == ng_b - 1;
+    const dim3 grid(G.count);
+#define FC_B(N) \
+  case N: launch_part(b, timed, slot, 5, f, l, k_fft_b_c<N>, grid, dim3(FC_T_B), 0, s, cj, ca, it); break;
+    switch (G.cls) {
+      FC_B(256) FC_B(512) FC_B(1024) FC_B(2048)
+      case 0:
+        launch_part(b, timed, slot, 5, f, l, k_fft_b<LDM_CAP_B, LDM_FFT_T_B>, grid, dim3(LDM_FFT_T_B), 0, s, cj, ca,
+                    it);
+        break;
+      default:
+        launch_part(b, timed, slot, 5, f, l, k_fft_b<LDM_CAP, LDM_FFT_T>, grid, dim3(LDM_FFT_T), 0, s, cj, ca, it);
+    }
+#undef FC_B
   }
   items += b->n_b;
-  if (b->n_c) launch(b, timed, slot, 6, k_fft_c, dim3(b->n_c), dim3(LDM_FFT_T), 0, s, cj, items, b->n_c);
+  for (int g = 0; g < ng_c; ++g) {
+    const auto& G = b->grp_c[g];
+    const int4* it = items + G.first;
+    const bool f = g == 0, l = g == ng_c - 1;
+    const dim3 grid(G.count);
+#define FC_C(N) \
+  case N: launch_part(b, timed, slot, 6, f, l, k_fft_c_c<N>, grid, dim3(FC_T), 0, s, cj, it, G.count); break;
+    switch (G.cls) {
+      FC_C(256) FC_C(320) FC_C(384) FC_C(512) FC_C(640) FC_C(768) FC_C(1024)
+      default: launch_part(b, timed, slot, 6, f, l, k_fft_c, grid, dim3(LDM_FFT_T), 0, s, cj, it, G.count);
+    }
+#undef FC_C
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }

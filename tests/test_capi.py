@@ -50,7 +50,9 @@ def test_plan_grid_matches_numpy(wmin, wmax, wstep):
         while x % p == 0:
             x //= p
     assert x == 1
-    assert L < 2 * nv * 1.06  # smooth lengths are dense
+    # 7-smooth lengths are dense (within 6 % of 2 N_v); a split with
+    # compile-time-sized passes is taken up to 1.35x the smallest of them
+    assert L < 2 * nv * 1.06 * 1.35
 
 
 def test_random_windows_grid_lengths():
