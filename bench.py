@@ -44,23 +44,28 @@ def parse():
     return ap.parse_args()
 
 
-def stage_bytes(n_lines, info, single_iso=True):
+def stage_bytes(n_lines, info, db):
     """Algorithmic HBM bytes per launch of each kernel of one request.
 
-    axis    : 12 B/line (g_air, g_self, n_air f32) [+ 12 B/line nu0+iso with several isotopologues]
-    fill    : 28 B/line (nu0 f64, S_ref, El, g_air, g_self, n_air f32) + 4*M*(N_v+1) LDM write
-    fft_a   : 4*M*(N_v+1) LDM read + 8*P*L packed-row spectra write
-    fft_b   : 8*P*L read + 8*L write
-    fft_c   : 8*L read + 4*N_v abscoeff write
+    axis        : 12 B/line (g_air, g_self, n_air f32) [+4 n_self] [+9 nu0 f64 + iso u8 with several isotopologues]
+    tile_bounds : 0 (a few dozen probes of nu0 per tile: latency, not bytes)
+    fill        : 28 B/line (nu0 f64; S_ref, El, g_air, g_self, n_air f32) [+4 n_self] [+4 delta] [+1 iso]
+                  + 4*M*(N_v+1) LDM write
+    fft_a       : 4*M*(N_v+1) LDM read + 8*P*L packed-row spectra write
+    fft_b       : 8*P*L read + 8*L write
+    fft_c       : 8*L read + 4*N_v abscoeff write
     """
     M = int(info.n_g * info.n_l)
     P = (M + 1) // 2
     L = int(info.fft_len)
     V = int(info.n_v) + 1
+    ns = 4 if db.has_nself else 0
+    per_fill = 28 + ns + (4 if db.has_delta else 0) + (1 if db.has_iso else 0)
     return {
-        "axis": n_lines * (12 if single_iso else 24),
+        "axis": n_lines * (12 + ns + (9 if db.has_iso else 0)),
         "finalize": 0,
-        "fill": n_lines * 28 + 4 * M * V,
+        "tile_bounds": 0,
+        "fill": n_lines * per_fill + 4 * M * V,
         "fft_a": 4 * M * V + 8 * P * L,
         "fft_b": 8 * P * L + 8 * L,
         "fft_c": 8 * L + 4 * int(info.n_v),
@@ -124,28 +129,50 @@ def bench_single(args):
     batch = engine.SpectrumBatch(db, [req])
     outs = batch.alloc_outputs()
     stream = torch.cuda.current_stream()
-    if args.graph:
-        batch.capture(outs)
-    step = (lambda: batch.replay(stream)) if args.graph else (lambda: batch.run(outs, stream))
+    eager = lambda: batch.run(outs, stream)  # noqa: E731
+
+    def timed_loop(step, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
     for _ in range(args.warmup):
-        step()
+        eager()
     torch.cuda.synchronize()
     inf = batch.info(0)
     if inf.status:
         raise RuntimeError(f"device status {inf.status}")
     n_win = int(inf.line_end - inf.line_begin)
-    if not args.graph:
-        batch.enable_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    stages = batch.stage_ms() if not args.graph else None
+    # 1) the timed steps: eager launches (or graph replays with --graph), no
+    #    per-kernel events
+    if args.graph:
+        batch.capture(outs)
+        replay = lambda: batch.replay(stream)  # noqa: E731
+        for _ in range(args.warmup):
+            replay()
+        dt = timed_loop(replay, args.steps) * args.steps
+    else:
+        dt = timed_loop(eager, args.steps) * args.steps
     ms = dt / args.steps * 1e3
     value = n_win * args.steps / dt
-    bytes_ = stage_bytes(n_win, inf, single_iso=not db.has_iso)
+    # 2) per-kernel device times from start/stop events of separate runs
+    batch.enable_timing(True)
+    timed_loop(eager, min(args.steps, 20))
+    stages = batch.stage_ms()
+    batch.enable_timing(False)
+    # 3) the other launch mode, for reference
+    if args.graph:
+        other = {"eager_ms_per_step": round(timed_loop(eager, args.steps) * 1e3, 4)}
+    else:
+        batch.capture(outs)
+        replay = lambda: batch.replay(stream)  # noqa: E731
+        for _ in range(args.warmup):
+            replay()
+        other = {"graph_ms_per_step": round(timed_loop(replay, args.steps) * 1e3, 4)}
+    bytes_ = stage_bytes(n_win, inf, db)
     roof = None
     if stages:
         dom = max(stages, key=stages.get)
@@ -192,6 +219,7 @@ def bench_single(args):
             "parallelism": "single-gpu",
         },
         "roofline": roof,
+        **other,
     }
     if not args.no_cpu_baseline:
         n_cpu = args.cpu_lines or {"co_5k": 5000, "co2_5m": 5_000_000, "h2o_100m": 2_000_000}.get(args.workload,

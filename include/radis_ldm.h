@@ -157,12 +157,13 @@ int radis_ldm_batch_info(radis_ldm_batch *b, int i, radis_ldm_request_info *info
 int radis_ldm_batch_workspace_bytes(const radis_ldm_batch *b, int64_t *bytes);
 void radis_ldm_batch_destroy(radis_ldm_batch *b);
 
-/* Device time per stage (axis, finalize, fill incl. tile bounds, fft_a,
- * fft_b, fft_c) in milliseconds, averaged over the runs made with timing
- * enabled since the previous call; each kernel is bracketed by its own
- * start/stop events (hipExtLaunchKernelGGL) on the launch stream.
+/* Device time per stage (axis, finalize, tile_bounds, fill, fft_a, fft_b,
+ * fft_c) in milliseconds, averaged over the runs made with timing enabled
+ * since the previous call; each stage is bracketed by start/stop events
+ * (hipExtLaunchKernelGGL) on the launch stream.  A stage whose kernel did not
+ * run (tile bounds reused) reports 0.
  * Replaces GPUApplication.get_timestamps (vulkan_compute_lib.py:973-1024). */
-#define RADIS_LDM_N_STAGES 6
+#define RADIS_LDM_N_STAGES 7
 int radis_ldm_batch_enable_timing(radis_ldm_batch *b, int enable);
 int radis_ldm_batch_stage_ms(radis_ldm_batch *b, float *ms_out);
 

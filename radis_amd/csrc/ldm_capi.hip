@@ -194,7 +194,7 @@ struct Req {
   int64_t stride = 0;
   int scale_log2 = 0;
   int bA = 0, bB = 0;
-  size_t off_ldm = 0, off_y = 0, off_b = 0, off_tiles = 0, off_rows = 0;
+  size_t off_ldm = 0, off_y = 0, off_b = 0, off_tiles = 0, off_rows = 0, off_axp = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
 };
 
@@ -233,7 +233,7 @@ struct radis_ldm_batch {
   bool timing = false;
   bool slot_timed[2] = {false, false};
   // start/stop events of every kernel of a timed run (tile bounds counted in the fill stage)
-  static constexpr int N_KERNELS = RADIS_LDM_N_STAGES + 1;
+  static constexpr int N_KERNELS = RADIS_LDM_N_STAGES;
   hipEvent_t st_ev[2][2 * N_KERNELS] = {};
   bool ev_used[2][N_KERNELS] = {};
   double stage_sum[RADIS_LDM_N_STAGES] = {0};
@@ -498,6 +498,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.row_a = (float*)(work + R.off_rows + (size_t)4 * R.m_cap);
   J.row_b = (float*)(work + R.off_rows + (size_t)8 * R.m_cap);
   J.row_inv_f0 = (float*)(work + R.off_rows + (size_t)12 * R.m_cap);
+  J.ax_part = (unsigned long long*)(work + R.off_axp);
   J.tw1 = R.tw1;
   J.tw2 = R.tw2;
   J.twl_hi = R.twl_hi;
@@ -548,6 +549,7 @@ int build_layout(radis_ldm_batch* b) {
     R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
     R.off_tiles = total; total = align_up(total + (size_t)R.n_tiles * 16, 256);
     R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
+    R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK) * 32, 256);
   }
   // --- twiddle tables, deduplicated by size
   std::map<int, size_t> sub_off;
@@ -710,8 +712,8 @@ void launch(radis_ldm_batch* b, bool timed, int slot, int kid, K kernel, dim3 gr
   launch_part(b, timed, slot, kid, true, true, kernel, grid, block, shmem, s, args...);
 }
 
-// stage of each kernel id: axis, finalize, tile bounds (fill), fill, fft_a, fft_b, fft_c
-const int KERNEL_STAGE[RADIS_LDM_N_STAGES + 1] = {0, 1, 2, 2, 3, 4, 5};
+// stage of each kernel id: axis, finalize, tile bounds, fill, fft_a, fft_b, fft_c
+const int KERNEL_STAGE[RADIS_LDM_N_STAGES] = {0, 1, 2, 3, 4, 5, 6};
 
 // The kernel sequence of one run (what a graph captures).
 int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool capturing = false) {
@@ -721,13 +723,13 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
   const LdmJob* cj = b->jobs_d;
   const LdmAxes* ca = b->axes_d;
   if (b->n_axis) launch(b, timed, slot, 0, k_axis, dim3(b->n_axis), dim3(256), 0, s, b->jobs_d, items, db.dev);
-  launch(b, timed, slot, 1, k_finalize, dim3(b->n), dim3(64), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
+  launch(b, timed, slot, 1, k_finalize, dim3(b->n), dim3(256), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
   items += b->n_axis;
   if (b->n_fill) {
     // Tile line bounds depend on the grid and, through the pressure-shift
     // margin, on p; without a shift column they are fixed per layout.
     if (db.has_delta || !b->tiles_valid) {
-      launch(b, timed, slot, 2, k_tile_bounds, dim3((b->n_fill + 255) / 256), dim3(256), 0, s, b->jobs_d, items,
+      launch(b, timed, slot, 2, k_tile_bounds, dim3((b->n_fill + 3) / 4), dim3(256), 0, s, b->jobs_d, items,
              b->n_fill, db.dev);
       // a captured launch has not run yet: only an eager run validates the bounds
       if (!capturing) b->tiles_valid = !db.has_delta;

@@ -87,6 +87,7 @@ struct LdmJob {
   const float2* tw2;             // exp(-2 pi i e/n2), e < n2
   const float2* twl_hi;          // W_L^(h*1024)
   const float2* twl_lo;          // W_L^(e), e < 1024
+  unsigned long long* ax_part;   // [chunks][4]: per-chunk width extremes (wl min, wl max, wg min, wg max) as bits
   // reduction targets and status, initialised by the host every run
   unsigned long long wl_min_bits, wl_max_bits, wg_min_bits, wg_max_bits;
   int status;

@@ -109,8 +109,9 @@ __device__ __forceinline__ unsigned long long double_to_bits(double d) {
 // ---------------------------------------------------------------------------
 // k_axis: min/max of the Lorentzian (and, for several isotopologues, the
 // Gaussian) FWHM over each request's window; radis/lbl/broadening.py:1823-1852.
-// Positive doubles order like their bit patterns, so the reduction uses
-// unsigned 64-bit atomics.  Item = (job, chunk of LDM_AX_CHUNK lines).
+// Positive doubles order like their bit patterns, so the extremes are reduced
+// as unsigned 64-bit integers: one partial per chunk, combined by k_finalize.
+// Item = (job, chunk of LDM_AX_CHUNK lines).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                               LdmDb db) {
@@ -191,14 +192,13 @@ __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const i
       gmin = red[2][k] < gmin ? red[2][k] : gmin;
       gmax = red[3][k] > gmax ? red[3][k] : gmax;
     }
-    if (e > a) {
-      atomicMin(&J.wl_min_bits, lmin);
-      atomicMax(&J.wl_max_bits, lmax);
-      if (!J.single_iso) {
-        atomicMin(&J.wg_min_bits, gmin);
-        atomicMax(&J.wg_max_bits, gmax);
-      }
-    }
+    // per-chunk partials, reduced by k_finalize (same-address global atomics
+    // from every workgroup would serialise in L2)
+    unsigned long long* P = J.ax_part + 4 * (long long)it.y;
+    P[0] = lmin;
+    P[1] = lmax;
+    P[2] = gmin;
+    P[3] = gmax;
   }
 }
 
@@ -215,13 +215,47 @@ __device__ __forceinline__ double voigt_ft64(double x, double wG, double wL) {
 // exactly as _init_w_axis (radis/lbl/broadening.py:1823-1837), the per-row
 // alias-fold counts and normalisations of the FT templates (1895-1926).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_finalize(LdmJob* __restrict__ jobs, LdmAxes* __restrict__ axes,
+__global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, LdmAxes* __restrict__ axes,
                                                  LdmResult* __restrict__ res, LdmDb db) {
   LdmJob& J = jobs[blockIdx.x];
   LdmAxes& A = axes[blockIdx.x];
   __shared__ double wgax[LDM_MAX_AX], wlax[LDM_MAX_AX];
   __shared__ double s_wg_min, s_wl_min;
   __shared__ int s_ng, s_nl, s_status;
+  __shared__ unsigned long long s_ext[4];
+  {
+    // reduce the per-chunk width extremes written by k_axis
+    unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
+    const long long nch = (J.i1 - J.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK;
+    for (long long c = threadIdx.x; c < nch; c += blockDim.x) {
+      const unsigned long long* P = J.ax_part + 4 * c;
+      lmin = P[0] < lmin ? P[0] : lmin;
+      lmax = P[1] > lmax ? P[1] : lmax;
+      gmin = P[2] < gmin ? P[2] : gmin;
+      gmax = P[3] > gmax ? P[3] : gmax;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      unsigned long long o;
+      o = __shfl_xor(lmin, off); lmin = o < lmin ? o : lmin;
+      o = __shfl_xor(lmax, off); lmax = o > lmax ? o : lmax;
+      o = __shfl_xor(gmin, off); gmin = o < gmin ? o : gmin;
+      o = __shfl_xor(gmax, off); gmax = o > gmax ? o : gmax;
+    }
+    __shared__ unsigned long long s_w[4][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_w[0][w] = lmin; s_w[1][w] = lmax; s_w[2][w] = gmin; s_w[3][w] = gmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+        lmin = s_w[0][k] < lmin ? s_w[0][k] : lmin;
+        lmax = s_w[1][k] > lmax ? s_w[1][k] : lmax;
+        gmin = s_w[2][k] < gmin ? s_w[2][k] : gmin;
+        gmax = s_w[3][k] > gmax ? s_w[3][k] : gmax;
+      }
+      s_ext[0] = lmin; s_ext[1] = lmax; s_ext[2] = gmin; s_ext[3] = gmax;
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     int status = 0, ng = 0, nl = 0;
     double wg_min = 0, wl_min = 0;
@@ -231,11 +265,11 @@ __global__ __launch_bounds__(64) void k_finalize(LdmJob* __restrict__ jobs, LdmA
         wg_min = fwhm_gauss(db.nu0[J.i0], J.dopp[1]);
         wg_max = fwhm_gauss(db.nu0[J.i1 - 1], J.dopp[1]);
       } else {
-        wg_min = bits_to_double(J.wg_min_bits);
-        wg_max = bits_to_double(J.wg_max_bits);
+        wg_min = bits_to_double(s_ext[2]);
+        wg_max = bits_to_double(s_ext[3]);
       }
-      wl_min = bits_to_double(J.wl_min_bits);
-      const double wl_max = bits_to_double(J.wl_max_bits);
+      wl_min = bits_to_double(s_ext[0]);
+      const double wl_max = bits_to_double(s_ext[1]);
       if (!(wg_min > 0.0) || !(wl_min > 0.0)) {
         status |= LDM_ST_WIDTH;
       } else {
@@ -313,6 +347,28 @@ __device__ __forceinline__ long long lower_bound_nu(const double* __restrict__ n
   return lo;
 }
 
+// Same result computed by a whole wave: 64 probes per round narrow the range
+// 64-fold (ballot + popcount), so ~log64(n) dependent loads instead of
+// log2(n).  Every lane of the wave must call it with the same arguments.
+__device__ __forceinline__ long long lower_bound_nu_wave(const double* __restrict__ nu0, long long lo, long long hi,
+                                                         double v) {
+  const int lane = threadIdx.x & 63;
+  // invariant: the answer lies in [lo, hi]
+  while (hi - lo > 64) {
+    const long long step = (hi - lo + 63) >> 6;
+    const long long idx = lo + (long long)(lane + 1) * step - 1;
+    const bool lt = idx < hi && nu0[idx] < v;
+    // probes are sorted: the c probes below v are lanes 0..c-1
+    const long long c = __popcll(__ballot(lt));
+    const long long nhi = lo + (c + 1) * step - 1;
+    lo += c * step;
+    hi = nhi < hi ? nhi : hi;
+  }
+  const long long idx = lo + lane;
+  const bool lt = idx < hi && nu0[idx] < v;
+  return lo + __popcll(__ballot(lt));
+}
+
 // ---------------------------------------------------------------------------
 // k_tile_bounds: for every fill tile [k_lo, k_hi) the line range that can
 // put a corner inside it: k0 in [k_lo-1, k_hi-1].  A line's np.interp
@@ -322,7 +378,8 @@ __device__ __forceinline__ long long lower_bound_nu(const double* __restrict__ n
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                                      int n_items, LdmDb db) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wave per item
+  const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (g >= n_items) return;
   const int4 it = items[g];
   LdmJob& J = jobs[it.x];
@@ -332,12 +389,14 @@ __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, 
   if (k_hi > J.v) k_hi = J.v;
   long long lo, hi;
   if (k_lo <= 1) lo = J.i0;
-  else lo = lower_bound_nu(db.nu0, J.i0, J.i1, J.wmin + (double)(k_lo - 1) * J.delta_w - J.shift_margin);
+  else lo = lower_bound_nu_wave(db.nu0, J.i0, J.i1, J.wmin + (double)(k_lo - 1) * J.delta_w - J.shift_margin);
   if (k_hi >= J.nv) hi = J.i1;
-  else hi = lower_bound_nu(db.nu0, J.i0, J.i1, J.wmin + (double)k_hi * J.delta_w + J.shift_margin);
+  else hi = lower_bound_nu_wave(db.nu0, lo, J.i1, J.wmin + (double)k_hi * J.delta_w + J.shift_margin);
   if (hi < lo) hi = lo;
-  J.tiles[2 * t] = lo;
-  J.tiles[2 * t + 1] = hi;
+  if ((threadIdx.x & 63) == 0) {
+    J.tiles[2 * t] = lo;
+    J.tiles[2 * t + 1] = hi;
+  }
 }
 
 // Everything the fill needs from one line:

@@ -124,6 +124,8 @@ class LineDatabase:
         self.nu0 = nu0
         self.n = int(nu0.size)
         self.has_iso = cols["iso"] is not None
+        self.has_delta = cols["delta"] is not None
+        self.has_nself = cols["n_self"] is not None
         self._keep = (nu0, cols, mm)
         st = _native.Lines()
         fp = lambda a: None if a is None else a.ctypes.data_as(POINTER(c_float))
