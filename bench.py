@@ -87,12 +87,35 @@ def read_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(workload, lines_all, cond, n_sample):
-    """FP64 NumPy restatement of RADIS's CPU LDM-fft path (the oracle) on a
-    bounded sample: the first ``n_sample`` lines of the same list, same grid."""
+def radis_cost_model_s(n_g, n_l, n_v):
+    """RADIS's own run-time estimate for broadening_method='fft',
+    optimization='simple' at equilibrium (lbl/factory.py:2139-2146)."""
+    import math
+
+    return 4.675e-08 * (1 + n_l * n_g) * n_v * math.log(n_v)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(workload, lines_all, cond, n_sample, dev, fft_len):
+    """FP64 NumPy restatement of RADIS's CPU LDM-fft path (the oracle) timed on
+    a bounded sample of the workload (the first ``n_sample`` lines by a seeded
+    choice, same grid), on one core and with all-core FFTs; on the same sample
+    the GPU result is checked against it: relative error to the peak at
+    L = 2 N_v (RADIS CPU semantics) and at the GPU's own L, and the number of
+    grid / log-width index mismatches."""
     import numpy as np
 
     from oracle import ldm_oracle as oracle
+    from radis_amd import engine
 
     n_sample = min(n_sample, len(lines_all))
     if n_sample == len(lines_all):
@@ -100,18 +123,43 @@ def cpu_baseline(workload, lines_all, cond, n_sample):
     else:
         idx = np.sort(np.random.default_rng(0).choice(len(lines_all), n_sample, replace=False))
         sub = lines_all.subset(idx)
+    d = sub.as_dict()
+    args = (d, cond.wmin, cond.wmax, cond.wstep, cond.Tgas, cond.pressure_bar, cond.mole_fraction, cond.dxG,
+            cond.dxL)
     t0 = time.perf_counter()
-    oracle.eq_abscoeff(sub.as_dict(), cond.wmin, cond.wmax, cond.wstep, cond.Tgas, cond.pressure_bar,
-                       cond.mole_fraction, cond.dxG, cond.dxL, Tref=cond.Tref)
-    dt = time.perf_counter() - t0
+    ref, det = oracle.eq_abscoeff(*args, Tref=cond.Tref, return_details=True)
+    dt1 = time.perf_counter() - t0
+    ncpu = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    oracle.eq_abscoeff(*args, Tref=cond.Tref, fft_workers=ncpu)
+    dtn = time.perf_counter() - t0
+    ref_L = oracle.eq_abscoeff(*args, Tref=cond.Tref, L=int(fft_len))
+    # the same sample on the GPU
+    db = engine.LineDatabase(dev, sub)
+    req = engine.request_from_conditions(db, cond, fft_len=int(fft_len))
+    b = engine.SpectrumBatch(db, [req])
+    gpu = b.results(b.run())[0]
+    t = b.line_terms(0)
+    mism = int(np.count_nonzero(t["k0"] != det["ki0"]) + np.count_nonzero(t["l0"] != det["li0"])
+               + np.count_nonzero(t["m0"] != det["mi0"])) if t["k0"].size == det["ki0"].size else -1
+    b.close()
+    db.close()
+    n_win = int(det["i1"] - det["i0"])
     return {
-        "value": n_sample / dt,
+        "value": round(n_win / dt1, 1),
         "unit": "lines/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{n_sample} of the {len(lines_all)} {workload} lines, full "
-                  f"{cond.wmin:g}-{cond.wmax:g} cm-1 grid at wstep {cond.wstep:g}, L=2*N_v, one spectrum, "
-                  f"{dt:.1f} s on {platform.processor() or platform.machine()}",
+        "sample": f"{n_sample} of the {len(lines_all)} {workload} lines ({n_win} in the window), full "
+                  f"{cond.wmin:g}-{cond.wmax:g} cm-1 grid at wstep {cond.wstep:g}, L=2*N_v, one spectrum: "
+                  f"{dt1:.2f} s on one core of {cpu_model()}",
+        "all_cores": {"value": round(n_win / dtn, 1), "cores": ncpu,
+                      "note": "same, with scipy.fft workers=os.cpu_count() (the scatter stays serial)"},
+        "radis_cost_model_s": round(radis_cost_model_s(det["NG"], det["NL"], det["Nv"]), 3),
+        "parity": {"sample_lines": n_win,
+                   "rel_err_L_2Nv": float(oracle.rel_peak_error(gpu, ref)),
+                   "rel_err_same_L": float(oracle.rel_peak_error(gpu, ref_L)),
+                   "fft_len": int(fft_len), "index_mismatches": mism, "tolerance": 1e-4},
     }
 
 
@@ -158,11 +206,17 @@ def bench_single(args):
         dt = timed_loop(eager, args.steps) * args.steps
     ms = dt / args.steps * 1e3
     value = n_win * args.steps / dt
-    # 2) per-kernel device times from start/stop events of separate runs
+    # 2) per-kernel device times from start/stop events of 20 separate runs;
+    #    the device time of a run is the sum of its stages (median reported)
     batch.enable_timing(True)
-    timed_loop(eager, min(args.steps, 20))
-    stages = batch.stage_ms()
+    per_run = []
+    for _ in range(20):
+        eager()
+        per_run.append(batch.stage_ms())
     batch.enable_timing(False)
+    stages = {k: float(np.mean([r[k] for r in per_run])) for k in per_run[0]}
+    device_ms = sorted(sum(r.values()) for r in per_run)
+    device_ms_median = 0.5 * (device_ms[9] + device_ms[10])
     # 3) the other launch mode, for reference
     if args.graph:
         other = {"eager_ms_per_step": round(timed_loop(eager, args.steps) * 1e3, 4)}
@@ -219,12 +273,14 @@ def bench_single(args):
             "parallelism": "single-gpu",
         },
         "roofline": roof,
+        "device_ms_median": round(device_ms_median, 4),
+        "device_lines_per_s_median": round(n_win / (device_ms_median * 1e-3), 1),
         **other,
     }
     if not args.no_cpu_baseline:
         n_cpu = args.cpu_lines or {"co_5k": 5000, "co2_5m": 5_000_000, "h2o_100m": 2_000_000}.get(args.workload,
                                                                                                  500_000)
-        out["cpu_baseline"] = cpu_baseline(args.workload, lines, cond, n_cpu)
+        out["cpu_baseline"] = cpu_baseline(args.workload, lines, cond, n_cpu, dev, inf.fft_len)
     batch.close()
     db.close()
     dev.close()

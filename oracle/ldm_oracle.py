@@ -210,15 +210,24 @@ def ldm_scatter(L, NG, NL, ki0, ki1, tvi, li0, li1, tGi, mi0, mi1, tLi, S):
     return LDM
 
 
-def ldm_apply_fft(LDM, templates, Nv, wstep):
+def ldm_apply_fft(LDM, templates, Nv, wstep, fft_workers=None):
     """Sum over (l, m) of rfft(LDM[:, l, m]) * template, irfft, crop, /wstep;
-    lbl/broadening.py:2523-2530."""
+    lbl/broadening.py:2523-2530.  ``fft_workers``: run the transforms with
+    scipy.fft on that many threads (same arithmetic, pocketfft underneath)
+    instead of numpy.fft's single thread."""
     NG, NL = LDM.shape[1], LDM.shape[2]
+    if fft_workers:
+        import scipy.fft as sfft
+
+        rfft = lambda a: sfft.rfft(a, workers=fft_workers)  # noqa: E731
+        irfft = lambda a: sfft.irfft(a, workers=fft_workers)  # noqa: E731
+    else:
+        rfft, irfft = np.fft.rfft, np.fft.irfft
     Ildm_FT = 1j * np.zeros(templates.shape[2])
     for l in range(NG):
         for m in range(NL):
-            Ildm_FT += np.fft.rfft(LDM[:, l, m]) * templates[l, m]
-    sumoflines = np.fft.irfft(Ildm_FT)[:Nv]
+            Ildm_FT += rfft(LDM[:, l, m]) * templates[l, m]
+    sumoflines = irfft(Ildm_FT)[:Nv]
     sumoflines /= wstep
     return sumoflines
 
@@ -275,6 +284,7 @@ def eq_abscoeff(
     L=None,
     folding_thresh=FOLDING_THRESH_DEFAULT,
     return_details=False,
+    fft_workers=None,
 ):
     """End-to-end abscoeff (cm-1) on the calculation grid, the way
     ``SpectrumFactory.eq_spectrum`` computes it with
@@ -304,7 +314,7 @@ def eq_abscoeff(
     mi0, mi1, tLi = get_indices(np.log(wL_dat), np.log(wL))
     LDM = ldm_scatter(L, wG.size, wL.size, ki0, ki1, tvi, li0, li1, tGi, mi0, mi1, tLi, S)
     templates = lineshape_templates(L, wstep, wG, wL, folding_thresh)
-    sumoflines = ldm_apply_fft(LDM, templates, Nv, wstep)
+    sumoflines = ldm_apply_fft(LDM, templates, Nv, wstep, fft_workers)
     abscoeff = sumoflines * number_density(mole_fraction, pressure_bar, Tgas)
     if return_details:
         return abscoeff, dict(

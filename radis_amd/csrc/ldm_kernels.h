@@ -587,10 +587,9 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 // LDS ds_add_f32 retires far slower than the 64-bit and integer LDS atomics
 // (measured ~3 cycles per lane per CU, which bounded this kernel), and the
 // FP64 sums are also more accurate.
-// Each lane walks its own contiguous run of the tile's nu-sorted lines: the
-// 64 lanes of an atomic instruction then target 64 distant grid points, so
-// same-address serialisation in LDS is rare.  The raw columns of the next
-// two lines are loaded while the current one is computed, and no LDS read
+// Consecutive lines go to consecutive threads, so loads are coalesced.  The
+// raw columns of a thread's next two lines are loaded while the current one
+// is computed, and no LDS read
 // sits on the common path (only atomics), so atomics retire in the background.
 // ---------------------------------------------------------------------------
 
@@ -612,20 +611,22 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   const LineConst C(J, db);
   const LineCols cols(db);
   const long long lo = J.tiles[2 * it.y], hi = J.tiles[2 * it.y + 1];
-  // each lane walks its own contiguous run of the tile's (nu-sorted) lines,
-  // so the 64 atomics of one instruction land on 64 distant grid points
-  const long long per = (hi - lo + LDM_FILL_T - 1) / LDM_FILL_T;
-  const long long g0 = lo + (long long)threadIdx.x * per;
-  const long long g1 = (g0 + per < hi) ? g0 + per : hi;
-  // first two lines in flight before the LDS set-up (indices clamped into
-  // [g0, g1 - 1], which is a valid line whenever the loop runs)
-  const long long glast = g1 - 1 > g0 ? g1 - 1 : g0;
+  // Thread t takes lines lo + t, lo + t + T, ...: every load instruction of
+  // a wave reads 64 consecutive lines.  (Giving each lane its own run of
+  // lines spreads a wave's loads over as many cache lines and pages as it has
+  // lanes, which measured up to 10x slower on dense lists; collisions of
+  // neighbouring lines on one LDS address cost far less.)
+  const long long hlast = hi - 1;
+  // first two lines in flight before the LDS set-up (indices clamped to the
+  // tile's last line, which is valid whenever the thread has a line)
   LineRaw n1, n2;
   IsoVals v1;
-  if (g0 < g1) {
-    cols.load(g0, n1);
+  const long long first = lo + threadIdx.x;
+  if (first < hi) {
+    cols.load(first, n1);
     iso_vals(J, n1.iso, v1);
-    cols.load(g0 + 1 < glast ? g0 + 1 : glast, n2);
+    const long long l1 = first + LDM_FILL_T;
+    cols.load(l1 < hlast ? l1 : hlast, n2);
   }
   {
     const int n2 = (M * Wp) >> 1;  // Wp is even: whole 16-byte pairs
@@ -636,12 +637,13 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   __syncthreads();
   const int wlen = (int)(k_hi - k_lo);
   const int sg = nl * Wp;
-  for (long long i = g0; i < g1; ++i) {
+  for (long long i = first; i < hi; i += LDM_FILL_T) {
     const LineRaw cur = n1;
     const IsoVals cv = v1;
     n1 = n2;
     iso_vals(J, n1.iso, v1);
-    cols.load(i + 2 < glast ? i + 2 : glast, n2);
+    const long long i2 = i + 2 * LDM_FILL_T;
+    cols.load(i2 < hlast ? i2 : hlast, n2);
     LineTerms t;
     line_terms(C, cur, cv, tab, ng, nl, t);
     const int c0 = t.k0 - (int)k_lo;  // column of corner k0 in the tile, in [-1, W-1] if touching
