@@ -87,13 +87,16 @@ typedef struct {
                                  otherwise wmin<=nu0<=wmax (lbl/loader.py:2707)          */
     int32_t flags;            /* RADIS_LDM_FP64_WIDTHS: log-width axis positions and
                                  fractions always in FP64 (default: FP32 with an exact
-                                 FP64 recomputation next to axis nodes)                  */
+                                 FP64 recomputation next to axis nodes);
+                                 RADIS_LDM_GENERIC_FFT: run-time-sized FFT passes even
+                                 where compile-time-sized ones exist                     */
     int32_t fft_n1;           /* 0 = planner; else the four-step split L = fft_n1 * n2
                                  (16 | fft_n1 <= 1024, 7-smooth cofactor)                */
     int32_t tile_w;           /* 0 = planner; else the fill tile width (power of 2)      */
 } radis_ldm_request;
 
 #define RADIS_LDM_FP64_WIDTHS 1
+#define RADIS_LDM_GENERIC_FFT 2
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {
@@ -106,6 +109,7 @@ typedef struct {
     int32_t scale_log2; /* output = stored value * 2**scale_log2   */
     int32_t n_g, n_l; /* device-computed LDM axis sizes (valid after a run has completed) */
     int32_t status;   /* device status of the last completed run (0 = ok) */
+    int32_t compiled_fft; /* 1: the FFT passes are the compile-time-sized kernels for (n1, n2) */
 } radis_ldm_request_info;
 
 /* --- context -------------------------------------------------------------- */

@@ -80,6 +80,7 @@ class Request(ctypes.Structure):
 
 
 FP64_WIDTHS = 1
+GENERIC_FFT = 2
 
 
 class RequestInfo(ctypes.Structure):
@@ -96,6 +97,7 @@ class RequestInfo(ctypes.Structure):
         ("n_g", c_int32),
         ("n_l", c_int32),
         ("status", c_int32),
+        ("compiled_fft", c_int32),
     ]
 
 

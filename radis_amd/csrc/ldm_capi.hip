@@ -301,7 +301,7 @@ int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int
   return bestL;
 }
 
-int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0) {
+int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow_fc = true) {
   const int64_t need = 2 * nv;
   if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);
   int a1 = 0, a2 = 0, s1 = 0, s2 = 0;
@@ -323,6 +323,7 @@ int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0) {
   int64_t bestc = -1;
   int c1 = 0, c2 = 0;
   for (int n1 : FC_N1) {
+    if (!allow_fc) break;
     if (only_n1 && n1 != only_n1) continue;
     for (int n2 : FC_N2) {
       const int64_t L = (int64_t)n1 * n2;
@@ -667,7 +668,7 @@ int plan_one(const radis_ldm_db& db, const radis_ldm_request& rq, Req& R) {
     return fail(RADIS_LDM_EINVAL, "fft_n1 %d: need 16 | fft_n1 <= %d", rq.fft_n1, N1_MAX);
   if (rq.tile_w && (rq.tile_w < MIN_TILE_W || rq.tile_w > MAX_TILE_W || (rq.tile_w & (rq.tile_w - 1))))
     return fail(RADIS_LDM_EINVAL, "tile_w %d: need a power of 2 in [%d, %d]", rq.tile_w, MIN_TILE_W, MAX_TILE_W);
-  rc = plan_fft(R.nv, rq.fft_len, R.plan, rq.fft_n1);
+  rc = plan_fft(R.nv, rq.fft_len, R.plan, rq.fft_n1, !(rq.flags & RADIS_LDM_GENERIC_FFT));
   if (rc) return rc;
   const bool lw = rq.line_hi > rq.line_lo;
   const double lo = lw ? rq.line_lo : rq.wmin, hi = lw ? rq.line_hi : rq.wmax;
@@ -1011,6 +1012,7 @@ int radis_ldm_request_bytes(const radis_ldm_db* db, const radis_ldm_request* req
     info->fft_len = R.plan.L;
     info->n1 = R.plan.n1;
     info->n2 = R.plan.n2;
+    info->compiled_fft = R.plan.fc ? 1 : 0;
     info->line_begin = R.i0;
     info->line_end = R.i1;
     info->m_cap = R.m_cap;
@@ -1170,6 +1172,7 @@ int radis_ldm_batch_info(radis_ldm_batch* b, int i, radis_ldm_request_info* info
   info->fft_len = R.plan.L;
   info->n1 = R.plan.n1;
   info->n2 = R.plan.n2;
+  info->compiled_fft = R.plan.fc ? 1 : 0;
   info->line_begin = R.i0;
   info->line_end = R.i1;
   info->m_cap = R.m_cap;

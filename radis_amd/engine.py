@@ -183,11 +183,14 @@ class LineDatabase:
 
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
-                 fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0):
+                 fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0,
+                 generic_fft=False):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
-    (lo, hi) overrides the line selection [wmin, wmax]."""
+    (lo, hi) overrides the line selection [wmin, wmax]; ``fft_n1``/``tile_w``
+    fix the FFT split and the fill tile width (default: the autotuner's cache,
+    else the planner); ``generic_fft`` selects the run-time-sized FFT passes."""
     r = _native.Request()
     r.wmin, r.wmax, r.wstep = float(wmin), float(wmax), float(wstep)
     r.Tgas, r.pressure_bar, r.mole_fraction = float(Tgas), float(pressure_bar), float(mole_fraction)
@@ -197,7 +200,7 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
         r.q_ratio[s] = float(q[s]) if s < q.size else 1.0
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
-    r.flags = _native.FP64_WIDTHS if fp64_widths else 0
+    r.flags = (_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
     if not (fft_n1 or tile_w):
         from . import autotune
 

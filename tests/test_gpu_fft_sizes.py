@@ -1,0 +1,74 @@
+"""Every compile-time-sized FFT split n1 x n2 against the run-time-sized
+(generic) passes on the same request, and against the FP64 oracle at the
+same FFT length.  The grid fills the transform (2 N_v just below L) so every
+bin of both passes carries signal through the lineshape product."""
+import numpy as np
+import pytest
+
+from oracle import ldm_oracle as oracle
+from radis_amd import engine, workloads
+
+FC_N1 = [256, 320, 384, 512, 640, 768, 1024]
+FC_N2 = [256, 512, 1024, 2048]
+WSTEP = 0.002
+WMIN = 2000.0
+
+
+def _lines(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    nu = np.sort(rng.uniform(lo, hi, n))
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    return workloads.LineList(
+        nu0=nu, S_ref=f32(10.0 ** rng.uniform(-24.0, -19.0, n)), El=f32(rng.uniform(0.0, 5000.0, n)),
+        g_air=f32(rng.uniform(0.03, 0.08, n)), g_self=f32(rng.uniform(0.05, 0.10, n)),
+        n_air=f32(rng.uniform(0.5, 0.8, n)), molar_mass=np.array([0.0, 44.0]), q_coef=(0.94, 1.1),
+        name="fft_sizes")
+
+
+def _grid(L):
+    """wmax such that 2 * len(arange(WMIN, wmax + WSTEP, WSTEP)) <= L, close to L."""
+    nv = L // 2 - 2
+    while True:
+        wmax = WMIN + (nv - 1) * WSTEP
+        size = np.arange(WMIN, wmax + WSTEP, WSTEP).size
+        if 2 * size <= L:
+            return wmax, size
+        nv -= 1
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = engine.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n1", FC_N1)
+def test_compiled_splits_match_generic_and_oracle(dev, n1):
+    for n2 in FC_N2:
+        L = n1 * n2
+        wmax, nv = _grid(L)
+        # lines over the first ~300 cm-1 (a few N_G rows), the rest of the grid
+        # holds the profile wings and zeros
+        lines = _lines(3000, WMIN + 1.0, min(WMIN + 300.0, wmax - 1.0), seed=n1 * 7 + n2)
+        db = engine.LineDatabase(dev, lines)
+        cond = dict(wmin=WMIN, wmax=wmax, wstep=WSTEP, Tgas=1200.0, pressure_bar=1.0, mole_fraction=0.2,
+                    dxG=0.2, dxL=0.2)
+        rc = engine.make_request(db, **cond, fft_len=L, fft_n1=n1)
+        rg = engine.make_request(db, **cond, fft_len=L, fft_n1=n1, generic_fft=True)
+        b = engine.SpectrumBatch(db, [rc, rg])
+        out = b.results(b.run())
+        ic, ig = b.info(0), b.info(1)
+        assert (ic.n1, ic.n2, ic.compiled_fft) == (n1, n2, 1), (n1, n2)
+        assert (ig.n1, ig.n2, ig.compiled_fft) == (n1, n2, 0), (n1, n2)
+        assert ic.n_v == nv and ic.fft_len == L
+        ref = oracle.eq_abscoeff(lines.as_dict(), WMIN, wmax, WSTEP, cond["Tgas"], cond["pressure_bar"],
+                                 cond["mole_fraction"], 0.2, 0.2, L=L)
+        e_cg = oracle.rel_peak_error(out[0], out[1])
+        e_ref = oracle.rel_peak_error(out[0], ref)
+        print(f"n1={n1} n2={n2} L={L}: compiled vs generic {e_cg:.2e}, vs oracle {e_ref:.2e}")
+        assert e_cg <= 2e-5, (n1, n2, e_cg)
+        assert e_ref <= 1e-4, (n1, n2, e_ref)
+        b.close()
+        db.close()
