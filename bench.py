@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--cpu-lines", type=int, default=0, help="lines in the CPU baseline sample (0 = auto)")
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson request stream")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the fill tile width and the FFT schedule per size class before timing "
+                         "(cache in a fresh temporary file)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -163,6 +166,25 @@ def cpu_baseline(workload, lines_all, cond, n_sample, dev, fft_len):
     }
 
 
+def run_autotune(db, reqs):
+    """Tune every size class of ``reqs`` into a fresh temporary cache (which
+    later ``make_request`` calls of this process read)."""
+    import tempfile
+
+    from radis_amd import autotune
+
+    os.environ["RADIS_AMD_TUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="radis_amd_tune_"), "tune.json")
+    t0 = time.perf_counter()
+    entries = autotune.tune_many(db, reqs)
+    dt = time.perf_counter() - t0
+    if len(entries) == 1:
+        (k, e), = entries.items()
+        return {"key": k, **{kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in e.items()},
+                "tuning_s": round(dt, 2)}
+    return {"size_classes": len(entries), "tuning_s": round(dt, 2),
+            "compiled_fft_classes": sum(1 for e in entries.values() if not e["generic"])}
+
+
 def bench_single(args):
     import numpy as np
     import torch
@@ -174,6 +196,10 @@ def bench_single(args):
     dev = engine.Device(0)
     db = engine.LineDatabase(dev, lines)
     req = engine.request_from_conditions(db, cond)
+    tuned = None
+    if args.autotune:
+        tuned = run_autotune(db, [req])
+        req = engine.request_from_conditions(db, cond)
     batch = engine.SpectrumBatch(db, [req])
     outs = batch.alloc_outputs()
     stream = torch.cuda.current_stream()
@@ -270,6 +296,9 @@ def bench_single(args):
             "x": cond.mole_fraction,
             "wstep": cond.wstep,
             "graph": bool(args.graph),
+            "compiled_fft": bool(inf.compiled_fft),
+            "tile_w": int(inf.tile_w),
+            "autotune": tuned,
             "parallelism": "single-gpu",
         },
         "roofline": roof,
@@ -313,6 +342,10 @@ def bench_batch(args):
     dev = engine.Device(0)
     db = engine.LineDatabase(dev, lines)
     reqs = [engine.request_from_conditions(db, c) for c in conds]
+    tuned = None
+    if args.autotune:
+        tuned = run_autotune(db, reqs)
+        reqs = [engine.request_from_conditions(db, c) for c in conds]
     batch = engine.SpectrumBatch(db, reqs)
     n_lines = sum(int(batch.infos[i].line_end - batch.infos[i].line_begin) for i in range(batch.n))
     stream = torch.cuda.Stream()
@@ -361,7 +394,8 @@ def bench_batch(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 indices, Boltzmann, Q ratio)",
         "data": "synthetic",
         "config": {"workload": "batch_256", "lines": len(lines), "requests": 256, "window_lines_total": n_lines,
-                   "wstep": 0.001, "graph": True, "workspace_bytes": ws, "parallelism": "single-gpu"},
+                   "wstep": 0.001, "graph": True, "workspace_bytes": ws, "autotune": tuned,
+                   "parallelism": "single-gpu"},
         "lines_per_s": round(n_lines * args.steps / dt, 1),
         "sequential_256": {"spectra_per_s": round(256 / dt_seq, 2), "ms": round(dt_seq * 1e3, 3),
                            "speedup_batched": round((dt_seq / (dt / args.steps)), 2)},

@@ -1,22 +1,37 @@
-"""Autotuner of the four-step FFT split and the fill tile width.
+"""Autotuner of a request's fill tile width and FFT schedule.
 
-For a request, candidate FFT splits ``L = n1 * n2`` whose length is within
-4 % of the smallest supported ``L >= 2 N_v``, and fill tile widths that fit
-LDS, are timed on the device (sum of the per-stage HIP-event times over a
-few runs); the fastest pair is stored in a JSON cache keyed by the grid
-length and LDM step sizes and used by later requests with the same key.
+The FFT schedule is the split L = n1 * n2 together with the pass kernels:
+compile-time-sized passes (fixed radix plans, for n1 in 2^a*{1,3,5} and n2 in
+2^a) or run-time-sized ones (any 7-smooth split).  The fill kernel and the
+FFT passes are separate kernels, so the two choices are tuned one after the
+other on their own stage times: the tile width on the fill time with the
+planner's FFT, then the FFT schedule on the time of the three FFT passes with
+the chosen tile.  Each candidate's time is the median over a few timed runs
+(per-kernel HIP events).
 
-Cache file: ``$RADIS_AMD_TUNE_CACHE`` or ``~/.cache/radis_amd/autotune.json``.
+Results are cached in a JSON file keyed by the planner's default FFT length
+and LDM row capacity (and the LDM steps), so requests of the same size class
+share one tuning: ``$RADIS_AMD_TUNE_CACHE`` or
+``~/.cache/radis_amd/autotune.json``.  ``engine.make_request`` applies a
+cached entry unless the caller fixes the schedule.
 """
 from __future__ import annotations
 
+import copy
 import json
-import math
 import os
 
 _CACHE = None
-_MAX_N1, _MAX_N2 = 1024, 8192
-_FILL_LDS = 60 * 1024
+_CACHE_PATH = None
+# sizes with compile-time-sized passes (mirrors FC_N1 / FC_N2 of the library;
+# a candidate is kept only if the planned request reports compiled_fft)
+FC_N1 = (256, 320, 384, 512, 640, 768, 1024)
+FC_N2 = (256, 512, 1024, 2048)
+TILE_WIDTHS = (16, 32, 64, 128, 256, 512)
+FILL_LDS_MAX = 60 * 1024  # bytes of FP64 tile cells per fill workgroup
+FFT_SLACK = 1.6           # compiled splits considered up to this factor above the smallest generic L
+FILL_STAGES = ("tile_bounds", "fill")
+FFT_STAGES = ("fft_a", "fft_b", "fft_c")
 
 
 def cache_path():
@@ -29,42 +44,88 @@ def cache_path():
 
 
 def _load():
-    global _CACHE
-    if _CACHE is None:
+    global _CACHE, _CACHE_PATH
+    path = cache_path()
+    if _CACHE is None or _CACHE_PATH != path:
         try:
-            with open(cache_path()) as f:
+            with open(path) as f:
                 _CACHE = json.load(f)
         except (OSError, ValueError):
             _CACHE = {}
+        _CACHE_PATH = path
     return _CACHE
 
 
 def _save():
     path = cache_path()
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         json.dump(_CACHE, f, indent=1, sort_keys=True)
     os.replace(tmp, path)
 
 
-def key_of(nv, dxG, dxL):
-    return f"nv={int(nv)}/dxG={dxG:.6g}/dxL={dxL:.6g}"
+def clear():
+    """Forget the cached entries (memory and file)."""
+    global _CACHE
+    _load()
+    _CACHE = {}
+    _save()
 
 
-def _grid_len(req):
-    if req.n_v:
-        return int(req.n_v)
-    return int(math.ceil(((req.wmax + req.wstep) - req.wmin) / req.wstep))
+def _generic_flag():
+    from . import _native
+
+    return _native.GENERIC_FFT
 
 
-def lookup(req):
-    """(fft_n1, tile_w) tuned for this request's key, else (0, 0)."""
+def _untuned(req):
+    r = copy.copy(req)
+    r.fft_len, r.fft_n1, r.tile_w = 0, 0, 0
+    r.flags = int(r.flags) & ~_generic_flag()
+    return r
+
+
+def _key_info(db, req):
+    _, info = db.request_bytes(_untuned(req))
+    return f"L={int(info.fft_len)}/mcap={int(info.m_cap)}/dxG={req.dxG:.6g}/dxL={req.dxL:.6g}", info
+
+
+def key_of(db, req):
+    """Size class of a request: the planner's FFT length and LDM row
+    capacity for it (no overrides), and the LDM steps."""
+    return _key_info(db, req)[0]
+
+
+def lookup(db, req):
+    """Tuned (fft_len, fft_n1, generic_fft, tile_w) for the request's size
+    class, or None."""
     c = _load()
     if not c:
-        return 0, 0
-    e = c.get(key_of(_grid_len(req), req.dxG, req.dxL))
-    return (int(e["fft_n1"]), int(e["tile_w"])) if e else (0, 0)
+        return None
+    e = c.get(key_of(db, req))
+    if not e:
+        return None
+    return int(e["fft_len"]), int(e["fft_n1"]), bool(e["generic"]), int(e["tile_w"])
+
+
+def apply(db, req):
+    """Set a cached schedule on ``req`` in place; returns whether one was
+    found.  Requests of one size class can differ slightly in N_v: the FFT
+    schedule is applied only if its length covers 2 N_v of this request (the
+    tile width always)."""
+    c = _load()
+    if not c:
+        return False
+    k, info = _key_info(db, req)
+    e = c.get(k)
+    if not e:
+        return False
+    req.tile_w = int(e["tile_w"])
+    if int(e["fft_len"]) >= 2 * int(info.n_v):
+        req.fft_len, req.fft_n1 = int(e["fft_len"]), int(e["fft_n1"])
+        req.flags = (int(req.flags) | _generic_flag()) if e["generic"] else (int(req.flags) & ~_generic_flag())
+    return True
 
 
 def _smooth16(limit):
@@ -79,62 +140,115 @@ def _smooth16(limit):
     return out
 
 
-def fft_candidates(nv, slack=1.04, max_n=6):
-    """n1 values whose L = n1 * n2 (smallest valid n2) is within ``slack`` of
-    the smallest, spread over the n1 range."""
+def fft_candidates(nv, n_generic=4):
+    """(fft_len, n1, generic) candidates for a grid of nv points: the smallest
+    7-smooth length with up to ``n_generic`` splits (run-time-sized passes),
+    and every compiled split with 2 nv <= L <= FFT_SLACK * that length."""
     need = 2 * nv
-    c2 = _smooth16(_MAX_N2)
-    cands = []
-    for n1 in _smooth16(_MAX_N1):
+    c2 = _smooth16(8192)
+    gen = []
+    for n1 in _smooth16(1024):
         n2min = -(-need // n1)
         n2 = next((v for v in c2 if v >= n2min), None)
         if n2 is not None:
-            cands.append((n1 * n2, n1))
-    if not cands:
+            gen.append((n1 * n2, n1))
+    if not gen:
         return []
-    Lmin = min(L for L, _ in cands)
-    good = sorted({n1 for L, n1 in cands if L <= slack * Lmin})
-    if len(good) > max_n:
-        idx = [round(i * (len(good) - 1) / (max_n - 1)) for i in range(max_n)]
-        good = [good[i] for i in sorted(set(idx))]
-    return good
+    Lmin = min(L for L, _ in gen)
+    best = sorted({n1 for L, n1 in gen if L == Lmin}, key=lambda n1: abs(n1 - 512))[:n_generic]
+    out = [(Lmin, n1, True) for n1 in best]
+    for n1 in FC_N1:
+        for n2 in FC_N2:
+            L = n1 * n2
+            if need <= L <= FFT_SLACK * Lmin:
+                out.append((L, n1, False))
+    return out
 
 
-def tune(db, req, reps=5, tile_widths=(64, 128, 256, 512, 1024), verbose=False):
-    """Time the candidates for ``req`` on the device; store and return the
-    best (fft_n1, tile_w, ms)."""
-    import copy
+def tile_candidates(m_cap):
+    return [w for w in TILE_WIDTHS if m_cap * (w + 2) * 8 <= FILL_LDS_MAX]
+
+
+def _time(db, req, stages, reps):
+    """Median over ``reps`` timed runs of the summed ``stages`` (ms) and the
+    planned info, or (None, None) if the schedule does not plan."""
+    import statistics
 
     from . import _native, engine
 
-    nv = _grid_len(req)
-    _, info = db.request_bytes(req)
-    m_cap = int(info.m_cap)
-    widths = [w for w in tile_widths if m_cap * (w + 2) * 8 <= _FILL_LDS] or [0]
-    results = []
-    for n1 in fft_candidates(nv) or [0]:
-        for w in widths:
-            r = copy.copy(req)
-            r.fft_n1, r.tile_w = n1, w
-            try:
-                b = engine.SpectrumBatch(db, [r])
-            except _native.LdmError:
-                continue
-            outs = b.alloc_outputs()
-            for _ in range(2):
-                b.run(outs)
-            b.enable_timing(True)
-            for _ in range(reps):
-                b.run(outs)
-            ms = sum(b.stage_ms().values())
-            b.close()
-            results.append((ms, n1, w))
+    try:
+        b = engine.SpectrumBatch(db, [req])
+    except _native.LdmError:
+        return None, None
+    try:
+        outs = b.alloc_outputs()
+        b.run(outs)
+        b.run(outs)
+        b.enable_timing(True)
+        ts = []
+        for _ in range(reps):
+            b.run(outs)
+            st = b.stage_ms()
+            ts.append(sum(st[s] for s in stages))
+        return statistics.median(ts), b.info(0)
+    finally:
+        b.close()
+
+
+def tune(db, req, reps=5, verbose=False):
+    """Time the candidates for ``req``; store and return the best entry."""
+    base = _untuned(req)
+    _, info = db.request_bytes(base)
+    nv, m_cap = int(info.n_v), int(info.m_cap)
+    # 1) fill tile width with the planner's FFT
+    fills = []
+    for w in tile_candidates(m_cap):
+        r = copy.copy(base)
+        r.tile_w = w
+        ms, _ = _time(db, r, FILL_STAGES, reps)
+        if ms is not None:
+            fills.append((ms, w))
             if verbose:
-                print(f"  n1={n1:5d} tile_w={w:5d}: {ms:.4f} ms", flush=True)
-    if not results:
-        return 0, 0, float("nan")
-    ms, n1, w = min(results)
+                print(f"  tile_w={w:4d}: fill {ms:.4f} ms", flush=True)
+    fill_ms, tile_w = min(fills) if fills else (float("nan"), int(info.tile_w))
+    # 2) FFT schedule with that tile
+    ffts = []
+    for L, n1, generic in fft_candidates(nv):
+        r = copy.copy(base)
+        r.tile_w, r.fft_len, r.fft_n1 = tile_w, L, n1
+        if generic:
+            r.flags = int(r.flags) | _generic_flag()
+        ms, inf = _time(db, r, FFT_STAGES, reps)
+        if ms is None or (not generic and not inf.compiled_fft):
+            continue
+        ffts.append((ms, L, n1, generic))
+        if verbose:
+            print(f"  L={L:8d} n1={n1:5d} {'generic ' if generic else 'compiled'}: fft {ms:.4f} ms", flush=True)
+    if ffts:
+        fft_ms, L, n1, generic = min(ffts)
+    else:
+        fft_ms, L, n1, generic = float("nan"), int(info.fft_len), int(info.n1), not bool(info.compiled_fft)
+    entry = {"fft_len": int(L), "fft_n1": int(n1), "generic": bool(generic), "tile_w": int(tile_w),
+             "fill_ms": fill_ms, "fft_ms": fft_ms, "candidates": len(fills) + len(ffts)}
     c = _load()
-    c[key_of(nv, req.dxG, req.dxL)] = {"fft_n1": n1, "tile_w": w, "ms": ms, "candidates": len(results)}
+    c[key_of(db, req)] = entry
     _save()
-    return n1, w, ms
+    return entry
+
+
+def tune_many(db, reqs, reps=5, verbose=False):
+    """Tune one representative request per size class (the first request of
+    each class without a cached entry); returns {key: entry}."""
+    done = {}
+    c = _load()
+    for r in reqs:
+        k = key_of(db, r)
+        if k in done:
+            continue
+        if k in c:
+            done[k] = c[k]
+            continue
+        done[k] = tune(db, r, reps=reps, verbose=verbose)
+        if verbose:
+            print(k, done[k], flush=True)
+    return done

@@ -201,13 +201,13 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
     r.flags = (_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
-    if not (fft_n1 or tile_w):
-        from . import autotune
-
-        fft_n1, tile_w = autotune.lookup(r)
     r.fft_n1, r.tile_w = int(fft_n1 or 0), int(tile_w or 0)
     if line_window is not None:
         r.line_lo, r.line_hi = float(line_window[0]), float(line_window[1])
+    if not (fft_len or fft_n1 or tile_w or generic_fft):
+        from . import autotune
+
+        autotune.apply(db, r)
     return r
 
 
