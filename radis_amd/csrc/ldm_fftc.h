@@ -47,9 +47,15 @@ constexpr int fc_pow2_floor(int n) { return n < 2 ? 1 : 2 * fc_pow2_floor(n >> 1
 constexpr int pad_c(int c) { return c + ((c >> 5) << 1); }
 
 // One Stockham DIT stage (see ldm_fft.h for the recursion) with compile-time
-// N, radix R, Ns, B and T.
-template <int N, int B, int T, int R, int NS, bool FWD>
-__device__ __forceinline__ void fc_stage(float2* __restrict__ buf, const float2* __restrict__ tw) {
+// N, radix R, Ns, B and T.  Element e = i*B + b is element i of column b.
+// IN_REG: the stage's inputs come from ld(s, q, e) instead of LDS (the first
+// stage, fed from registers or global memory; its LDS writes need the
+// buffer free, which the caller guarantees).  OUT_REG: its outputs go to
+// st(s, q, e, value) instead of LDS (the last stage; no trailing barrier).
+// (s, q) = (thread pass, butterfly leg), the same for a given e in every
+// call, so callers can keep per-element state in registers indexed by them.
+template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST>
+__device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st) {
   constexpr int NR = N / R;
   constexpr int ITEMS = NR * B;
   constexpr int IT = (ITEMS + T - 1) / T;
@@ -64,7 +70,10 @@ __device__ __forceinline__ void fc_stage(float2* __restrict__ buf, const float2*
   for (int s = 0; s < IT; ++s) {
     const int it = threadIdx.x + s * T;
     if (ITEMS % T == 0 || it < ITEMS) {
-      if constexpr (RD_LIN) {
+      if constexpr (IN_REG) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[s][q] = ld(s, q, it + q * NR * B);
+      } else if constexpr (RD_LIN) {
         const float2* src = buf + pad_idx(it);
 #pragma unroll
         for (int q = 0; q < R; ++q) v[s][q] = src[pad_c(q * NR * B)];
@@ -85,7 +94,8 @@ __device__ __forceinline__ void fc_stage(float2* __restrict__ buf, const float2*
       dft<R, FWD>(v[s]);
     }
   }
-  __syncthreads();
+  // LDS inputs must all be read before any output overwrites them
+  if constexpr (!IN_REG && !OUT_REG) __syncthreads();
 #pragma unroll
   for (int s = 0; s < IT; ++s) {
     const int it = threadIdx.x + s * T;
@@ -93,7 +103,10 @@ __device__ __forceinline__ void fc_stage(float2* __restrict__ buf, const float2*
       const int j = it >> LB, b = it & (B - 1);
       const int k = j % NS;
       const int e0 = (((j - k) * R + k) << LB) + b;
-      if constexpr (WR_LIN) {
+      if constexpr (OUT_REG) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) st(s, q, e0 + q * NS * B, v[s][q]);
+      } else if constexpr (WR_LIN) {
         float2* dst = buf + pad_idx(e0);
 #pragma unroll
         for (int q = 0; q < R; ++q) dst[pad_c(q * NS * B)] = v[s][q];
@@ -103,16 +116,45 @@ __device__ __forceinline__ void fc_stage(float2* __restrict__ buf, const float2*
       }
     }
   }
-  __syncthreads();
+  if constexpr (!OUT_REG) __syncthreads();
 }
 
-template <int N, int B, int T, bool FWD, int S = 0>
-__device__ __forceinline__ void fc_fft(float2* __restrict__ buf, const float2* __restrict__ tw) {
-  if constexpr (S < fc_nstages(N)) {
-    fc_stage<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD>(buf, tw);
-    fc_fft<N, B, T, FWD, S + 1>(buf, tw);
+struct FcNone {
+  __device__ float2 operator()(int, int, int) const { return make_float2(0.0f, 0.0f); }
+  __device__ void operator()(int, int, int, float2) const {}
+};
+
+// Transform of the B columns in buf.  IN_REG: the first stage reads its
+// inputs through ld (then calls after_first()); OUT_REG: the last stage
+// hands its outputs to st.  A transform with both set touches LDS only
+// between its stages.
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, int S = 0, class LD, class ST, class AF>
+__device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
+                                          AF& after_first) {
+  constexpr int NST = fc_nstages(N);
+  if constexpr (S < NST) {
+    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1>(buf, tw, ld,
+                                                                                                        st);
+    if constexpr (S == 0) after_first();
+    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, S + 1>(buf, tw, ld, st, after_first);
   }
 }
+
+template <int N, int B, int T, bool FWD>
+__device__ __forceinline__ void fc_fft(float2* __restrict__ buf, const float2* __restrict__ tw) {
+  FcNone none;
+  auto nop = [] {};
+  fc_fft_io<N, B, T, FWD, false, false>(buf, tw, none, none, nop);
+}
+
+// (thread pass, leg) extents of a transform's first and last stages, for
+// callers that keep per-element values in registers
+template <int N, int B, int T>
+struct FcShape {
+  static constexpr int R0 = fc_radix(N, 0), IT0 = (N / R0 * B + T - 1) / T;
+  static constexpr int RL = fc_radix(N, fc_nstages(N) - 1), ITL = (N / RL * B + T - 1) / T;
+  static constexpr int ITEMS0 = N / R0 * B, ITEMSL = N / RL * B;
+};
 
 // stage the forward twiddle table W_N^e, e < N, into LDS
 template <int N, int T>
@@ -147,42 +189,46 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_a_c(const LdmJob* __restrict__ 
   const int n2_0 = it.z * B;
   const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
   const bool has_b = (2 * p + 1) < axes[it.x].m;
-  const float* rb = ra + J.ldm_stride;
+  const float* rb = has_b ? ra + J.ldm_stride : ra;
   const int V = (int)J.v;
+  // read by the second stage, after the first stage's barrier
   fc_load_tw<N1, FC_T>(s_tw, J.tw1);
-#pragma unroll
-  for (int s = 0; s < CAPE / FC_T; ++s) {
-    const int e = threadIdx.x + s * FC_T;
-    const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-    float xa = 0.0f, xb = 0.0f;
-    if (n < V) {
-      xa = ra[n];
-      if (has_b) xb = rb[n];
-    }
-    buf[pad_idx(e)] = make_float2(xa, xb);
-  }
-  __syncthreads();
-  fc_fft<N1, B, FC_T, true>(buf, s_tw);
   float2* Y = J.ybuf + (long long)p * J.L;
-#pragma unroll
-  for (int s = 0; s < CAPE / FC_T; ++s) {
-    const int e = threadIdx.x + s * FC_T;
+  // first stage straight from the two LDM rows (zero beyond V), last stage
+  // straight to Y with the W_L twiddle: LDS holds only the middle stages
+  auto ld = [&](int, int, int e) -> float2 {
+    const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
+    const bool in = n < V;
+    const int nc = in ? n : 0;
+    const float xa = ra[nc], xb = rb[nc];
+    return make_float2(in ? xa : 0.0f, (in && has_b) ? xb : 0.0f);
+  };
+  auto st = [&](int, int, int e, float2 z) {
     const int k1 = e >> LB, n2 = n2_0 + (e & (B - 1));
-    Y[(long long)k1 * N2 + n2] = cmul(buf[pad_idx(e)], twiddle_L(J, (long long)n2 * k1));
-  }
+    Y[(long long)k1 * N2 + n2] = cmul(z, twiddle_L(J, (long long)n2 * k1));
+  };
+  auto nop = [] {};
+  fc_fft_io<N1, B, FC_T, true, true, true>(buf, s_tw, ld, st, nop);
 }
 
 // ---------------------------------------------------------------------------
 // Pass B with N2 fixed (see k_fft_b): B = FC_CAP_B / N2 rows k1 per workgroup.
+// Per pair the forward transform's first stage reads the pair's block from
+// registers (fetched during the previous pair) and its last stage multiplies
+// by the lineshape FTs and accumulates in registers; only the accumulated
+// spectrum goes through LDS into the inverse transform, whose last stage
+// writes B with the W_L twiddle.
 // ---------------------------------------------------------------------------
 template <int N2>
 __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
                                                       const LdmAxes* __restrict__ axes,
                                                       const int4* __restrict__ items) {
   constexpr int CAP = FC_CAP_B, T = FC_T_B;
-  constexpr int B = fc_rows_b(N2), LB = fc_log2(B), LN2 = fc_log2(N2);
-  constexpr int NACC = CAP / T;
+  constexpr int B = fc_rows_b(N2), LB = fc_log2(B);
   static_assert(fc_pow2(N2), "pass B rows are powers of two");
+  using SH = FcShape<N2, B, T>;
+  constexpr int NR0 = N2 / SH::R0;
+  constexpr int NSL = fc_ns(N2, fc_nstages(N2) - 1);
   __shared__ float2 buf[LdsSize<CAP>::value];
   __shared__ float2 s_tw[N2];
   const int4 it = items[blockIdx.x];
@@ -193,61 +239,75 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   const long long L = J.L;
   const int P = axes[it.x].p, M = axes[it.x].m;
   fc_load_tw<N2, T>(s_tw, J.tw2);
-  float2 acc[NACC];
+  float2 acc[SH::ITL][SH::RL];
 #pragma unroll
-  for (int s = 0; s < NACC; ++s) acc[s] = make_float2(0.0f, 0.0f);
-  // element e of a pair's block: row b = e >> log2(N2), column n2 (fastest
-  // in global memory), stored transposed at LDS [n2][b]; the next pair's
-  // block is fetched into registers while this one is transformed
+  for (int s = 0; s < SH::ITL; ++s)
+#pragma unroll
+    for (int q = 0; q < SH::RL; ++q) acc[s][q] = make_float2(0.0f, 0.0f);
+  // pair p's block: rows k1_0 .. k1_0+B-1 of Y_p, element e = n2*B + b
   const float2* Yb = J.ybuf + (long long)k1_0 * N2;
-  float2 pre[NACC];
-  if (P > 0) {
+  float2 pre[SH::IT0][SH::R0];
+  auto prefetch = [&](int p) {
+    const float2* Yp = Yb + (long long)p * L;
 #pragma unroll
-    for (int s = 0; s < NACC; ++s) pre[s] = Yb[threadIdx.x + s * T];
-  }
+    for (int s = 0; s < SH::IT0; ++s) {
+      const int t = threadIdx.x + s * T;
+      if (SH::ITEMS0 % T == 0 || t < SH::ITEMS0) {
+#pragma unroll
+        for (int q = 0; q < SH::R0; ++q) {
+          const int e = t + q * NR0 * B;
+          pre[s][q] = Yp[(e & (B - 1)) * N2 + (e >> LB)];
+        }
+      }
+    }
+  };
+  if (P > 0) prefetch(0);
   for (int p = 0; p < P; ++p) {
-#pragma unroll
-    for (int s = 0; s < NACC; ++s) {
-      const int e = threadIdx.x + s * T;
-      buf[pad_idx(((e & (N2 - 1)) << LB) + (e >> LN2))] = pre[s];
-    }
-    __syncthreads();
-    if (p + 1 < P) {
-      const float2* Yn = Yb + (long long)(p + 1) * L;
-#pragma unroll
-      for (int s = 0; s < NACC; ++s) pre[s] = Yn[threadIdx.x + s * T];
-    }
-    fc_fft<N2, B, T, true>(buf, s_tw);
+    // the previous pair's last stage has finished reading LDS
+    if (p > 0) __syncthreads();
     const int ra = 2 * p, rb = 2 * p + 1;
-#pragma unroll
-    for (int s = 0; s < NACC; ++s) {
-      const int e = threadIdx.x + s * T;
+    auto ld = [&](int s, int q, int) -> float2 { return pre[s][q]; };
+    auto st = [&](int s, int q, int e, float2 z) {
       const int k2 = e >> LB, b = e & (B - 1);
       const long long k = (long long)(k1_0 + b) + (long long)N1 * k2;
       const long long kk = (k <= L - k) ? k : (L - k);
       const float fa = lineshape_ft(J, ra, kk);
       const float fb = (rb < M) ? lineshape_ft(J, rb, kk) : 0.0f;
-      const float2 z = buf[pad_idx(e)];
-      acc[s].x += z.x * fa + z.y * fb;
-      acc[s].y += z.y * fa - z.x * fb;
-    }
-    __syncthreads();
+      acc[s][q].x += z.x * fa + z.y * fb;
+      acc[s][q].y += z.y * fa - z.x * fb;
+    };
+    auto next = [&] {
+      if (p + 1 < P) prefetch(p + 1);
+    };
+    fc_fft_io<N2, B, T, true, true, true>(buf, s_tw, ld, st, next);
   }
-#pragma unroll
-  for (int s = 0; s < NACC; ++s) buf[pad_idx(threadIdx.x + s * T)] = acc[s];
   __syncthreads();
-  fc_fft<N2, B, T, false>(buf, s_tw);
-  float2* Bo = J.bbuf + (long long)k1_0 * N2;
+  // the accumulated spectrum into LDS, in the forward last stage's element order
 #pragma unroll
-  for (int s = 0; s < NACC; ++s) {
-    const int e = threadIdx.x + s * T;
-    const int b = e >> LN2, n2 = e & (N2 - 1);
-    Bo[e] = cmulc(buf[pad_idx((n2 << LB) + b)], twiddle_L(J, (long long)n2 * (k1_0 + b)));
+  for (int s = 0; s < SH::ITL; ++s) {
+    const int t = threadIdx.x + s * T;
+    if (SH::ITEMSL % T == 0 || t < SH::ITEMSL) {
+      const int j = t >> LB, b = t & (B - 1);
+      const int k = j % NSL;
+      const int e0 = (((j - k) * SH::RL + k) << LB) + b;
+#pragma unroll
+      for (int q = 0; q < SH::RL; ++q) buf[pad_idx(e0 + q * NSL * B)] = acc[s][q];
+    }
   }
+  __syncthreads();
+  float2* Bo = J.bbuf + (long long)k1_0 * N2;
+  auto st_out = [&](int, int, int e, float2 z) {
+    const int b = e & (B - 1), n2 = e >> LB;
+    Bo[b * N2 + n2] = cmulc(z, twiddle_L(J, (long long)n2 * (k1_0 + b)));
+  };
+  FcNone none;
+  auto nop = [] {};
+  fc_fft_io<N2, B, T, false, false, true>(buf, s_tw, none, st_out, nop);
 }
 
 // ---------------------------------------------------------------------------
-// Pass C with N1 fixed (see k_fft_c).
+// Pass C with N1 fixed (see k_fft_c): first stage straight from B, last
+// stage straight to the output.
 // ---------------------------------------------------------------------------
 template <int N1>
 __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items,
@@ -271,18 +331,12 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ 
     return;
   }
   fc_load_tw<N1, FC_T>(s_tw, J.tw1);
-#pragma unroll
-  for (int s = 0; s < CAPE / FC_T; ++s) {
-    const int e = threadIdx.x + s * FC_T;
-    buf[pad_idx(e)] = J.bbuf[(long long)(e >> LB) * N2 + n2_0 + (e & (B - 1))];
-  }
-  __syncthreads();
-  fc_fft<N1, B, FC_T, false>(buf, s_tw);
   const float sc = J.out_scale;
-#pragma unroll
-  for (int s = 0; s < CAPE / FC_T; ++s) {
-    const int e = threadIdx.x + s * FC_T;
+  auto ld = [&](int, int, int e) -> float2 { return J.bbuf[(long long)(e >> LB) * N2 + n2_0 + (e & (B - 1))]; };
+  auto st = [&](int, int, int e, float2 z) {
     const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-    if (n < nv) J.out[n] = buf[pad_idx(e)].x * sc;
-  }
+    if (n < nv) J.out[n] = z.x * sc;
+  };
+  auto nop = [] {};
+  fc_fft_io<N1, B, FC_T, false, true, true>(buf, s_tw, ld, st, nop);
 }
