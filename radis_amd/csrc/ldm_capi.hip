@@ -194,7 +194,7 @@ struct Req {
   int64_t stride = 0;
   int scale_log2 = 0;
   int bA = 0, bB = 0;
-  size_t off_ldm = 0, off_y = 0, off_b = 0, off_tiles = 0, off_rows = 0, off_axp = 0;
+  size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
 };
 
@@ -494,7 +494,6 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.ybuf = (float2*)(work + R.off_y);
   J.bbuf = (float2*)(work + R.off_b);
   J.out = out;
-  J.tiles = (long long*)(work + R.off_tiles);
   J.nfold = (int*)(work + R.off_rows);
   J.row_a = (float*)(work + R.off_rows + (size_t)4 * R.m_cap);
   J.row_b = (float*)(work + R.off_rows + (size_t)8 * R.m_cap);
@@ -548,7 +547,6 @@ int build_layout(radis_ldm_batch* b) {
     R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
     R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
     R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
-    R.off_tiles = total; total = align_up(total + (size_t)R.n_tiles * 16, 256);
     R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
     R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK) * 32, 256);
   }
@@ -730,8 +728,8 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
     // Tile line bounds depend on the grid and, through the pressure-shift
     // margin, on p; without a shift column they are fixed per layout.
     if (db.has_delta || !b->tiles_valid) {
-      launch(b, timed, slot, 2, k_tile_bounds, dim3((b->n_fill + 3) / 4), dim3(256), 0, s, b->jobs_d, items,
-             b->n_fill, db.dev);
+      launch(b, timed, slot, 2, k_tile_bounds, dim3((b->n_fill + 3) / 4), dim3(256), 0, s, b->jobs_d,
+             b->items_d + (items - b->items_d), b->n_fill, db.dev);
       // a captured launch has not run yet: only an eager run validates the bounds
       if (!capturing) b->tiles_valid = !db.has_delta;
     }

@@ -78,7 +78,6 @@ struct LdmJob {
   float2* ybuf;                  // [(m_cap+1)/2][L]
   float2* bbuf;                  // [L]
   float* out;                    // [nv]
-  long long* tiles;              // [2*n_tiles] fill line bounds
   int* nfold;                    // [m_cap]
   float* row_a;                  // [m_cap] (pi*wG_l)^2/(4 ln2)
   float* row_b;                  // [m_cap] pi*wL_m

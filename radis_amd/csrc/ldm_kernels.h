@@ -376,7 +376,7 @@ __device__ __forceinline__ long long lower_bound_nu_wave(const double* __restric
 // the largest pressure shift (plus two grid steps for rounding) is a
 // superset; k_fill filters each line by its exact k0.  Item = (job, tile).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
+__global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, int4* __restrict__ items,
                                                      int n_items, LdmDb db) {
   // one wave per item
   const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -393,9 +393,11 @@ __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, 
   if (k_hi >= J.nv) hi = J.i1;
   else hi = lower_bound_nu_wave(db.nu0, lo, J.i1, J.wmin + (double)k_hi * J.delta_w + J.shift_margin);
   if (hi < lo) hi = lo;
+  // the range goes into the tile's own work item (offsets from the window
+  // start; windows hold < 2^31 lines), so the fill reads it with the item
   if ((threadIdx.x & 63) == 0) {
-    J.tiles[2 * t] = lo;
-    J.tiles[2 * t + 1] = hi;
+    items[g].z = (int)(lo - J.i0);
+    items[g].w = (int)(hi - J.i0);
   }
 }
 
@@ -572,7 +574,14 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
   if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
   const double s64 = (double)r.S * iv.q * C.i_scale;
-  const float stim = expm1f(C.c_stim_T * nuf) / expm1f(C.c_stim_R * nuf);
+  // (1 - exp(-c2 nu/T)) / (1 - exp(-c2 nu/Tref)), radis/lbl/base.py:2664-2674:
+  // for |x| >= 0.1 the difference 1 - e^x loses under 4 bits, so the fast
+  // exponential and division (~2 ulp) keep ~1e-6 relative accuracy; smaller
+  // arguments (far-infrared lines, hot gas) take expm1f
+  const float xT = C.c_stim_T * nuf, xR = C.c_stim_R * nuf;
+  float stim;
+  if (xT < -0.1f && xR < -0.1f) stim = __fdividef(1.0f - __expf(xT), 1.0f - __expf(xR));
+  else stim = expm1f(xT) / expm1f(xR);
   t.I = boltzmann_scale(s64, C.c_boltz * (double)r.El) * stim;
 }
 
@@ -610,7 +619,7 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   if (k_hi > J.v) k_hi = J.v;
   const LineConst C(J, db);
   const LineCols cols(db);
-  const long long lo = J.tiles[2 * it.y], hi = J.tiles[2 * it.y + 1];
+  const long long lo = J.i0 + it.z, hi = J.i0 + it.w;  // written by k_tile_bounds
   // Thread t takes lines lo + t, lo + t + T, ...: every load instruction of
   // a wave reads 64 consecutive lines.  (Giving each lane its own run of
   // lines spreads a wave's loads over as many cache lines and pages as it has
