@@ -163,6 +163,26 @@ __device__ __forceinline__ void fc_load_tw(float2* __restrict__ s_tw, const floa
   for (int e = threadIdx.x; e < N; e += T) s_tw[e] = tw[e];
 }
 
+// Twiddles W_L^(m * (j0 + s*DS + q*DQ)) of one thread's last-stage outputs
+// (s < IT thread passes, q < R legs; m fixed per thread) as
+// base * A^s * C^q from three table values, so an output costs one complex
+// product instead of two table gathers.  The longest product chain has
+// IT + R - 2 factors (~1e-7 rounding each).  Every exponent is below L.
+template <int IT, int R>
+struct TwProg {
+  float2 ws[IT], wq[R];
+  __device__ __forceinline__ void init(const LdmJob& J, long long m, long long j0, long long DS, long long DQ) {
+    ws[0] = twiddle_L(J, m * j0);
+    const float2 A = twiddle_L(J, m * DS), C = twiddle_L(J, m * DQ);
+#pragma unroll
+    for (int s = 1; s < IT; ++s) ws[s] = cmul(ws[s - 1], A);
+    wq[0] = make_float2(1.0f, 0.0f);
+#pragma unroll
+    for (int q = 1; q < R; ++q) wq[q] = cmul(wq[q - 1], C);
+  }
+  __device__ __forceinline__ float2 operator()(int s, int q) const { return cmul(ws[s], wq[q]); }
+};
+
 // ---------------------------------------------------------------------------
 // Pass A with N1 fixed (see k_fft_a): B = fc_cols_a(N1) columns per workgroup.
 // ---------------------------------------------------------------------------
@@ -194,6 +214,11 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_a_c(const LdmJob* __restrict__ 
   // read by the second stage, after the first stage's barrier
   fc_load_tw<N1, FC_T>(s_tw, J.tw1);
   float2* Y = J.ybuf + (long long)p * J.L;
+  // output k1 of this thread's last-stage element (s, q): j0 + s*(T/B) + q*NS
+  using SH = FcShape<N1, B, FC_T>;
+  constexpr int NSL = fc_ns(N1, fc_nstages(N1) - 1);
+  TwProg<SH::ITL, SH::RL> tw;
+  tw.init(J, n2_0 + (threadIdx.x & (B - 1)), threadIdx.x >> LB, FC_T >> LB, NSL);
   // first stage straight from the two LDM rows (zero beyond V), last stage
   // straight to Y with the W_L twiddle: LDS holds only the middle stages
   auto ld = [&](int, int, int e) -> float2 {
@@ -203,9 +228,9 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_a_c(const LdmJob* __restrict__ 
     const float xa = ra[nc], xb = rb[nc];
     return make_float2(in ? xa : 0.0f, (in && has_b) ? xb : 0.0f);
   };
-  auto st = [&](int, int, int e, float2 z) {
+  auto st = [&](int s, int q, int e, float2 z) {
     const int k1 = e >> LB, n2 = n2_0 + (e & (B - 1));
-    Y[(long long)k1 * N2 + n2] = cmul(z, twiddle_L(J, (long long)n2 * k1));
+    Y[(long long)k1 * N2 + n2] = cmul(z, tw(s, q));
   };
   auto nop = [] {};
   fc_fft_io<N1, B, FC_T, true, true, true>(buf, s_tw, ld, st, nop);
@@ -261,18 +286,29 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       }
     }
   };
+  // frequencies of this thread's bins (forward last-stage element (s, q):
+  // k2 = j + q*NS, j = (tid + s*T) >> LB), folded to [0, L/2]
+  float xs[SH::ITL][SH::RL], xrs[SH::ITL][SH::RL];
+#pragma unroll
+  for (int s = 0; s < SH::ITL; ++s) {
+    const int t = threadIdx.x + s * T;
+#pragma unroll
+    for (int q = 0; q < SH::RL; ++q) {
+      const long long k = (long long)(k1_0 + (t & (B - 1))) + (long long)N1 * ((t >> LB) + q * NSL);
+      const long long kk = (k <= L - k) ? k : (L - k);
+      xs[s][q] = (float)kk * J.inv_Lws;
+      xrs[s][q] = (float)(L / 2 - kk) * J.inv_Lws;
+    }
+  }
   if (P > 0) prefetch(0);
   for (int p = 0; p < P; ++p) {
     // the previous pair's last stage has finished reading LDS
     if (p > 0) __syncthreads();
     const int ra = 2 * p, rb = 2 * p + 1;
     auto ld = [&](int s, int q, int) -> float2 { return pre[s][q]; };
-    auto st = [&](int s, int q, int e, float2 z) {
-      const int k2 = e >> LB, b = e & (B - 1);
-      const long long k = (long long)(k1_0 + b) + (long long)N1 * k2;
-      const long long kk = (k <= L - k) ? k : (L - k);
-      const float fa = lineshape_ft(J, ra, kk);
-      const float fb = (rb < M) ? lineshape_ft(J, rb, kk) : 0.0f;
+    auto st = [&](int s, int q, int, float2 z) {
+      const float fa = lineshape_ft_x(J, ra, xs[s][q], xrs[s][q]);
+      const float fb = (rb < M) ? lineshape_ft_x(J, rb, xs[s][q], xrs[s][q]) : 0.0f;
       acc[s][q].x += z.x * fa + z.y * fb;
       acc[s][q].y += z.y * fa - z.x * fb;
     };
@@ -296,9 +332,12 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   }
   __syncthreads();
   float2* Bo = J.bbuf + (long long)k1_0 * N2;
-  auto st_out = [&](int, int, int e, float2 z) {
+  // output n2 of this thread's last-stage element (s, q): j0 + s*(T/B) + q*NS, row k1 fixed
+  TwProg<SH::ITL, SH::RL> tw;
+  tw.init(J, k1_0 + (threadIdx.x & (B - 1)), threadIdx.x >> LB, T >> LB, NSL);
+  auto st_out = [&](int s, int q, int e, float2 z) {
     const int b = e & (B - 1), n2 = e >> LB;
-    Bo[b * N2 + n2] = cmulc(z, twiddle_L(J, (long long)n2 * (k1_0 + b)));
+    Bo[b * N2 + n2] = cmulc(z, tw(s, q));
   };
   FcNone none;
   auto nop = [] {};

@@ -757,6 +757,19 @@ __global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_a(const LdmJob* __restrict
   }
 }
 
+// Same, from the bin's frequency x = kk/(L wstep) and xr = (L/2 - kk)/(L wstep)
+// (both computed by the caller once per bin).
+__device__ __forceinline__ float lineshape_ft_x(const LdmJob& J, int r, float x, float xr) {
+  const float a = J.row_a[r], bco = J.row_b[r];
+  float v = __expf(-(a * x * x) - bco * x);
+  const int nf = J.nfold[r];
+  for (int n = 1; n <= nf; ++n) {
+    const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xr) : ((float)n * J.half_inv_ws + x);
+    v += __expf(-(a * xa * xa) - bco * xa);
+  }
+  return v * J.row_inv_f0[r];
+}
+
 // Folded, normalised FT lineshape of LDM row r at bin k (|k| folded to
 // [0, L/2]); radis/lbl/broadening.py:1895-1926.
 __device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, long long kk) {
