@@ -171,6 +171,27 @@ void radis_ldm_batch_destroy(radis_ldm_batch *b);
 int radis_ldm_batch_enable_timing(radis_ldm_batch *b, int enable);
 int radis_ldm_batch_stage_ms(radis_ldm_batch *b, float *ms_out);
 
+/* Derived spectral quantities written by the last pass next to abscoeff
+ * (device pointers of n_v floats, NULL = not wanted), with the same
+ * arithmetic as radis/lbl/factory.py:1289-1297 and
+ * radis/spectrum/equations.py:21-44 (radiance from
+ * radis/phys/blackbody.py:35-74), evaluated in FP64 per grid point:
+ *   absorbance    = abscoeff * path_length   (stored with the request's
+ *                   2**-scale_log2 factor, like abscoeff)
+ *   emissivity    = -expm1(-absorbance)
+ *   transmittance = 1 - emissivity
+ *   radiance      = emissivity * Planck(1e7/nu nm, Tgas)   mW/cm2/sr/nm
+ * The reference computes these on the host after the GPU run. */
+typedef struct {
+    float *absorbance, *transmittance, *emissivity, *radiance;
+    double path_length; /* cm */
+} radis_ldm_derived;
+
+/* Set (or clear with NULL) request i's derived outputs for the following
+ * runs and graph replays.  Replaces the host post-processing of
+ * eq_spectrum_gpu (radis/lbl/factory.py:1284-1297). */
+int radis_ldm_batch_set_derived(radis_ldm_batch *b, int i, const radis_ldm_derived *d);
+
 /* --- diagnostics ---------------------------------------------------------- */
 /* Per-line values the fill kernel scatters, for every line of request i's
  * window (line_end - line_begin entries, device pointers), computed with the

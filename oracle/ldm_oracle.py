@@ -29,6 +29,8 @@ scatter                lbl/broadening.py:2340-2346, 2478-2485;
                        misc/arrays.py:472-502 (``numpy_add_at``)
 FFT apply              lbl/broadening.py:2523-2530
 density, abscoeff      lbl/factory.py:908-918
+derived quantities     lbl/factory.py:1289-1297, spectrum/equations.py:21-44,
+                       phys/blackbody.py:35-74, phys/convert.py:120-122
 =====================  ==========================================================
 
 The one generalisation: the FFT length ``L`` is a parameter.  RADIS uses
@@ -323,6 +325,30 @@ def eq_abscoeff(
             wG_dat=wG_dat, wL_dat=wL_dat,
         )
     return abscoeff
+
+
+def cm2nm(w_cm1):
+    """Vacuum wavelength (nm) of a wavenumber (cm-1); phys/convert.py:120-122."""
+    return 1 / w_cm1 * 1e9 / 100
+
+
+def planck(lambda_nm, T):
+    """Blackbody spectral radiance in mW/cm2/sr/nm; phys/blackbody.py:35-74."""
+    lbd = np.asarray(lambda_nm, dtype=np.float64) * 1e-9
+    with np.errstate(over="ignore"):
+        ipl = (2 * h * c**2 / lbd**5) * (1 / (np.exp(h * c / (lbd * k_b * T)) - 1))
+    return ipl * 1e-10
+
+
+def derived_quantities(wavenumber, abscoeff, path_length, Tgas):
+    """absorbance, emissivity, transmittance and radiance from abscoeff
+    (cm-1) along a path of ``path_length`` cm at ``Tgas`` K;
+    lbl/factory.py:1289-1297 with spectrum/equations.py:21-44."""
+    absorbance = np.asarray(abscoeff, dtype=np.float64) * path_length
+    emissivity = -np.expm1(-absorbance)
+    transmittance = 1 - emissivity
+    radiance = emissivity * planck(cm2nm(np.asarray(wavenumber, dtype=np.float64)), Tgas)
+    return dict(absorbance=absorbance, emissivity=emissivity, transmittance=transmittance, radiance=radiance)
 
 
 def residual_L2(a, b):

@@ -101,6 +101,16 @@ class RequestInfo(ctypes.Structure):
     ]
 
 
+class Derived(ctypes.Structure):
+    _fields_ = [
+        ("absorbance", c_void_p),
+        ("transmittance", c_void_p),
+        ("emissivity", c_void_p),
+        ("radiance", c_void_p),
+        ("path_length", c_double),
+    ]
+
+
 _PROTOTYPES = {
     "radis_ldm_last_error": (c_char_p, []),
     "radis_ldm_abi_version": (c_int, []),
@@ -116,6 +126,7 @@ _PROTOTYPES = {
     "radis_ldm_batch_create": (c_int, [c_void_p, c_void_p, c_int, POINTER(Request), POINTER(c_void_p)]),
     "radis_ldm_batch_set_request": (c_int, [c_void_p, c_int, POINTER(Request)]),
     "radis_ldm_batch_run": (c_int, [c_void_p, POINTER(c_void_p), c_void_p]),
+    "radis_ldm_batch_set_derived": (c_int, [c_void_p, c_int, POINTER(Derived)]),
     "radis_ldm_batch_graph_capture": (c_int, [c_void_p, POINTER(c_void_p), c_void_p]),
     "radis_ldm_batch_graph_replay": (c_int, [c_void_p, c_void_p]),
     "radis_ldm_batch_info": (c_int, [c_void_p, c_int, POINTER(RequestInfo), c_void_p]),

@@ -184,6 +184,7 @@ const double FC_SLACK = 1.35;
 
 struct Req {
   radis_ldm_request r{};
+  radis_ldm_derived derived{};  // all null: none
   int64_t nv = 0;
   double delta_w = 0;
   FftPlan plan;
@@ -494,6 +495,14 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.ybuf = (float2*)(work + R.off_y);
   J.bbuf = (float2*)(work + R.off_b);
   J.out = out;
+  J.d_abs = R.derived.absorbance;
+  J.d_trans = R.derived.transmittance;
+  J.d_emis = R.derived.emissivity;
+  J.d_rad = R.derived.radiance;
+  J.derived = (J.d_abs || J.d_trans || J.d_emis || J.d_rad) ? 1 : 0;
+  J.path_length = R.derived.path_length;
+  J.Tgas = r.Tgas;
+  J.scale_log2 = R.scale_log2;
   J.nfold = (int*)(work + R.off_rows);
   J.row_a = (float*)(work + R.off_rows + (size_t)4 * R.m_cap);
   J.row_b = (float*)(work + R.off_rows + (size_t)8 * R.m_cap);
@@ -1110,6 +1119,14 @@ int radis_ldm_batch_run(radis_ldm_batch* b, float* const* out_dev, void* stream)
   b->pending[slot] = true;
   b->last_slot = slot;
   b->next_slot = slot ^ 1;
+  return 0;
+}
+
+int radis_ldm_batch_set_derived(radis_ldm_batch* b, int i, const radis_ldm_derived* d) {
+  if (!b) return fail(RADIS_LDM_EINVAL, "null batch");
+  if (i < 0 || i >= b->n) return fail(RADIS_LDM_EINVAL, "request index %d out of range [0, %d)", i, b->n);
+  if (d && !(d->path_length >= 0.0)) return fail(RADIS_LDM_EINVAL, "path_length must be >= 0");
+  b->reqs[i].derived = d ? *d : radis_ldm_derived{};
   return 0;
 }
 

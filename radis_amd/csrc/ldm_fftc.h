@@ -374,7 +374,7 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ 
   auto ld = [&](int, int, int e) -> float2 { return J.bbuf[(long long)(e >> LB) * N2 + n2_0 + (e & (B - 1))]; };
   auto st = [&](int, int, int e, float2 z) {
     const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-    if (n < nv) J.out[n] = z.x * sc;
+    if (n < nv) store_output(J, n, z.x * sc);
   };
   auto nop = [] {};
   fc_fft_io<N1, B, FC_T, false, true, true>(buf, s_tw, ld, st, nop);

@@ -78,6 +78,10 @@ struct LdmJob {
   float2* ybuf;                  // [(m_cap+1)/2][L]
   float2* bbuf;                  // [L]
   float* out;                    // [nv]
+  // derived outputs (null = not wanted): absorbance, transmittance, emissivity, radiance
+  float *d_abs, *d_trans, *d_emis, *d_rad;
+  double path_length, Tgas;
+  int derived, scale_log2;
   int* nfold;                    // [m_cap]
   float* row_a;                  // [m_cap] (pi*wG_l)^2/(4 ln2)
   float* row_b;                  // [m_cap] pi*wL_m

@@ -877,6 +877,31 @@ __global__ __launch_bounds__(T, 768 / T) void k_fft_b(const LdmJob* __restrict__
   }
 }
 
+// abscoeff (times 2^-scale_log2) at grid point n and, when requested, the
+// derived quantities (FP64): radis/lbl/factory.py:1289-1297,
+// radis/spectrum/equations.py:21-44, radis/phys/blackbody.py:35-74.
+#define LDM_H_SI 6.62607015e-34      /* radis/phys/constants.py:22,33,39 */
+#define LDM_C_SI 2.99792458e8
+#define LDM_KB_SI 1.3806490e-23
+__device__ __forceinline__ void store_output(const LdmJob& J, int n, float v) {
+#pragma clang fp contract(off)
+  J.out[n] = v;
+  if (!J.derived) return;
+  const double a_scaled = (double)v * J.path_length;   // absorbance times 2^-scale_log2
+  const double A = ldexp(a_scaled, J.scale_log2);
+  const double emis = -expm1(-A);
+  if (J.d_abs) J.d_abs[n] = (float)a_scaled;
+  if (J.d_emis) J.d_emis[n] = (float)emis;
+  if (J.d_trans) J.d_trans[n] = (float)(1.0 - emis);
+  if (J.d_rad) {
+    const double nu = J.wmin + (double)n * J.delta_w;   // grid value (numpy.arange rule)
+    const double lbd = (1.0 / nu * 1e9 / 100.0) * 1e-9;  // m (radis/phys/convert.py:120-122)
+    const double pl = (2.0 * LDM_H_SI * (LDM_C_SI * LDM_C_SI) / (lbd * lbd * lbd * lbd * lbd)) *
+                      (1.0 / (exp(LDM_H_SI * LDM_C_SI / (lbd * LDM_KB_SI * J.Tgas)) - 1.0));
+    J.d_rad[n] = (float)(emis * (pl * 1e-10));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // k_fft_c: item = (job, column block cb).  y[n2 + N2 n1] =
 // sum_k1 B[k1][n2] W_N1^(-n1 k1); abscoeff[n] = Re(y[n]) / (L wstep) for
@@ -906,7 +931,7 @@ __global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_c(const LdmJob* __restrict
   for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
     const int n1 = e >> bs, b = e & (B - 1);
     const long long n = n2_0 + b + (long long)N2 * n1;
-    if (n < J.nv) J.out[n] = buf[pad_idx(e)].x * sc;
+    if (n < J.nv) store_output(J, (int)n, buf[pad_idx(e)].x * sc);
   }
 }
 

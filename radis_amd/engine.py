@@ -307,6 +307,41 @@ class SpectrumBatch:
         check(rc)
         return self._graph_outs
 
+    DERIVED = ("absorbance", "transmittance", "emissivity", "radiance")
+
+    def set_derived(self, i, path_length, which=DERIVED):
+        """Have the last pass also write request i's derived quantities
+        (``which`` of absorbance, transmittance, emissivity, radiance) for a
+        path of ``path_length`` cm, on every following run and replay.
+        Returns the float32 device tensors (absorbance stored times
+        2**-scale_log2, like abscoeff).  ``which=()`` turns them off."""
+        torch = _torch()
+        bad = set(which) - set(self.DERIVED)
+        if bad:
+            raise ValueError(f"unknown derived quantities {sorted(bad)}")
+        nv = self.n_v(i) if 0 <= i < self.n else 0  # the library reports a bad index
+        tens = {k: torch.empty(nv, dtype=torch.float32, device="cuda") for k in which}
+        d = _native.Derived(*[c_void_p(tens[k].data_ptr()) if k in tens else None for k in self.DERIVED],
+                            float(path_length))
+        check(self._lib.radis_ldm_batch_set_derived(self._h, int(i), byref(d) if which else None))
+        if not hasattr(self, "_derived"):
+            self._derived = {}
+        self._derived[i] = tens  # the library holds raw pointers: keep the tensors alive
+        return tens
+
+    def derived_results(self, i):
+        """Request i's derived quantities from the last run as float64 numpy
+        arrays (absorbance rescaled by 2**scale_log2); synchronises."""
+        tens = getattr(self, "_derived", {}).get(i, {})
+        inf = self.info(i)
+        out = {}
+        for k, t in tens.items():
+            a = t[: inf.n_v].double().cpu().numpy()
+            if k == "absorbance" and inf.scale_log2:
+                a = np.ldexp(a, int(inf.scale_log2))
+            out[k] = a
+        return out
+
     def _check_outs(self, outs):
         torch = _torch()
         if len(outs) != self.n:

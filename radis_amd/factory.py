@@ -6,8 +6,8 @@ radis/lbl/calc.py:37, 1014-1028).
 Only the parts of those callers that belong to the GPU path are reproduced:
 unit handling (K, bar, cm), the calculation grid, the synthesis itself, the
 derived quantities computed right after it (absorbance, emissivity,
-transmittance, Planck radiance; radis/lbl/factory.py:1284-1297) and the
-``NwL``/``NwG`` conditions.  Line lists are handed in directly (``lines=``);
+transmittance, Planck radiance; radis/lbl/factory.py:1284-1297, written here
+by the last GPU pass next to abscoeff) and the ``NwL``/``NwG`` conditions.  Line lists are handed in directly (``lines=``);
 databank download and file parsing are not part of this package.
 """
 from __future__ import annotations
@@ -19,52 +19,30 @@ import numpy as np
 from . import engine
 from .engine import DXG_DEFAULT, DXL_DEFAULT, FOLDING_THRESH_DEFAULT
 
-# radis/phys/constants.py
-_H, _C, _KB = 6.62607015e-34, 2.99792458e8, 1.3806490e-23
-
-
-def planck_nm(lambda_nm, T):
-    """Blackbody radiance in mW/cm2/sr/nm at vacuum wavelength (nm);
-    radis/phys/blackbody.py:35-74."""
-    lbd = lambda_nm * 1e-9
-    with np.errstate(over="ignore"):
-        ipl = (2 * _H * _C**2 / lbd**5) * (1 / (np.exp(_H * _C / (lbd * _KB * T)) - 1))
-    return ipl * 1e-10
-
-
-def calc_radiance(wavenumber, emissivity, Tgas):
-    """radiance_noslit (mW/cm2/sr/nm) = emissivity * Planck(1e7/nu, T);
-    radis/spectrum/equations.py:21-44 with radis/phys/convert.py cm2nm."""
-    return emissivity * planck_nm(1e7 / wavenumber, Tgas)
-
-
 class Spectrum:
     """Result of ``eq_spectrum_gpu`` holding the GPU application until
     ``exit_gpu`` (radis/spectrum/spectrum.py:176-179, 366, 528)."""
 
+    DERIVED_NAMES = {"absorbance": "absorbance", "emissivity": "emissivity_noslit",
+                     "transmittance": "transmittance_noslit", "radiance": "radiance_noslit"}
     UNITS = {"abscoeff": "cm-1", "absorbance": "", "emissivity_noslit": "", "transmittance_noslit": "",
              "radiance_noslit": "mW/cm2/sr/nm", "wavenumber": "cm-1"}
 
-    def __init__(self, wavenumber, abscoeff, conditions, gpu_app=None, name=None):
+    def __init__(self, wavenumber, abscoeff, derived, conditions, gpu_app=None, name=None):
         self._w = wavenumber
         self.conditions = dict(conditions)
         self.gpu_app = gpu_app
         self.name = name
         self._q = {}
-        self._set_abscoeff(abscoeff)
+        self._set_quantities(abscoeff, derived)
 
-    def _set_abscoeff(self, abscoeff):
-        # radis/lbl/factory.py:1284-1297
-        L = self.conditions["path_length"]
-        absorbance = abscoeff * L
-        emissivity = -np.expm1(-absorbance)
-        self._q = {
-            "abscoeff": abscoeff,
-            "absorbance": absorbance,
-            "emissivity_noslit": emissivity,
-            "transmittance_noslit": 1 - emissivity,
-            "radiance_noslit": calc_radiance(self._w, emissivity, self.conditions["Tgas"]),
-        }
+    def _set_quantities(self, abscoeff, derived):
+        """abscoeff and the quantities the last GPU pass derived from it
+        (radis/lbl/factory.py:1284-1297), cropped to ``woutrange``."""
+        a, b = self.conditions["woutrange"]
+        self._q = {"abscoeff": abscoeff[a:b]}
+        for k, name in self.DERIVED_NAMES.items():
+            self._q[name] = derived[k][a:b]
 
     def get_vars(self):
         return list(self._q)
@@ -92,11 +70,11 @@ class Spectrum:
                 self.conditions[k] = v
         self.conditions["Tvib"] = self.conditions["Trot"] = self.conditions["Tgas"]
         abscoeff, _ = self.gpu_app.iterate(self.conditions["pressure"], self.conditions["Tgas"],
-                                           self.conditions["mole_fraction"])
+                                           self.conditions["mole_fraction"],
+                                           path_length=self.conditions["path_length"])
         NwL, NwG = self.gpu_app.get_griddims()
         self.conditions["NwL"], self.conditions["NwG"] = NwL, NwG
-        self._set_abscoeff(abscoeff[self.conditions["woutrange"][0]:self.conditions["woutrange"][1]]
-                           .astype(np.float64))
+        self._set_quantities(abscoeff, self.gpu_app.derived)
         self.conditions["calculation_time"] = time.perf_counter() - t0
         return self.get(var, wunit=wunit, Iunit=Iunit, copy=False)[1]
 
@@ -169,7 +147,7 @@ class SpectrumFactory:
         cols = self._line_columns()
         w = self.wavenumber
         app = _LineApp(self, cols, w.size, device_id)
-        abscoeff, _ = app.iterate(self.pressure, float(Tgas), self.mole_fraction)
+        abscoeff, _ = app.iterate(self.pressure, float(Tgas), self.mole_fraction, path_length=self.path_length)
         NwL, NwG = app.get_griddims()
         conditions = dict(
             Tgas=float(Tgas), Tvib=float(Tgas), Trot=float(Tgas), pressure=self.pressure,
@@ -179,7 +157,7 @@ class SpectrumFactory:
             diluents={"air": 1 - self.mole_fraction}, gpu_backend="hip", spectral_points=int(w.size),
             add_at_used="gpu-backend", woutrange=(0, int(w.size)), calculation_time=time.perf_counter() - t0,
         )
-        s = Spectrum(w, abscoeff.astype(np.float64), conditions, gpu_app=None if exit_gpu else app, name=name)
+        s = Spectrum(w, abscoeff, app.derived, conditions, gpu_app=None if exit_gpu else app, name=name)
         if exit_gpu:
             app.__del__()
         return s
@@ -200,21 +178,31 @@ class _LineApp:
         self._batch = None
         self._outs = None
         self._griddims = (0, 0)
+        self._path_length = None
+        self.derived = {}
 
     def _req(self, p, T, x):
         sf = self.sf
         return engine.make_request(self._db, sf.wavenum_min, sf.wavenum_max, sf.wstep, T, p, x, sf.dxG, sf.dxL,
                                    sf.Tref, sf.folding_thresh)
 
-    def iterate(self, p, T, x):
+    def iterate(self, p, T, x, path_length=None):
+        """abscoeff (cm-1) at (p bar, T K, x); with ``path_length`` (cm) the
+        last pass also writes absorbance, emissivity, transmittance and
+        radiance, left in ``self.derived``."""
         req = self._req(p, T, x)
         if self._batch is None:
             self._batch = engine.SpectrumBatch(self._db, [req])
             self._outs = self._batch.alloc_outputs()
         else:
             self._batch.set_request(0, req)
+        if path_length != self._path_length:
+            self._batch.set_derived(0, path_length, which=engine.SpectrumBatch.DERIVED if path_length is not None
+                                    else ())
+            self._path_length = path_length
         self._batch.run(self._outs)
         out = self._batch.results(self._outs)[0]
+        self.derived = self._batch.derived_results(0)
         inf = self._batch.info(0)
         self._griddims = (int(inf.n_l), int(inf.n_g))
         return out, {}

@@ -33,6 +33,15 @@ C2 = 6.62607015e-34 * 2.99792458e8 / 1.3806490e-23 * 100  # hc/k in cm K
 TREF = 296.0  # radis/gpu/params.py:146
 
 
+def s_ref_from_s0(S0, El, v0, Q_Tref, Tref=TREF):
+    """Tabulated-form intensity S(Tref) from the Einstein-A form
+    ``S0 = Ia g' A / (8 pi c v0^2)`` (radis/lbl/base.py:2437-2441): times the
+    Boltzmann factor and stimulated-emission term at Tref over Q(Tref)."""
+    El = np.asarray(El, dtype=np.float64)
+    v0 = np.asarray(v0, dtype=np.float64)
+    return np.asarray(S0, dtype=np.float64) * np.exp(-C2 * El / Tref) * (1 - np.exp(-C2 * v0 / Tref)) / Q_Tref
+
+
 def next_fast_len_even(n):
     """FFT length the planner uses for a grid needing at least ``n`` points
     (counterpart of radis/gpu/gpu.py:22-29, for this library's radices)."""
@@ -60,8 +69,7 @@ class gpuApp:
         gamma_arr = np.asarray(gamma_arr, dtype=np.float64)
         El64 = np.asarray(El, dtype=np.float64)
         Q = np.array([self.Q_intp_list[int(i)](TREF) for i in range(len(self.Q_intp_list))], dtype=np.float64)
-        S_ref = (np.asarray(S0, dtype=np.float64) * np.exp(-C2 * El64 / TREF)
-                 * (1 - np.exp(-C2 * v0 / TREF)) / Q[iso])
+        S_ref = s_ref_from_s0(S0, El64, v0, Q[iso])
         # grid: vmin + i*dv for i < Nv; every line handed in is used (lines
         # outside the grid are clamped onto its edges, as np.interp does)
         self.wmax = self.vmin + (self.Nv - 1) * self.dv
@@ -86,11 +94,14 @@ class gpuApp:
                             Tref=TREF, folding_thresh=self.folding_thresh, n_v=self.Nv,
                             line_window=(-np.inf, np.inf))
 
-    def iterate(self, p, T, mole_fraction, verbose=0, l=1.0, slit_FWHM=0.0):
+    def iterate(self, p, T, mole_fraction, verbose=0, l=1.0, slit_FWHM=0.0, derived=False):
         """abscoeff (cm-1, float32[Nv]) and per-stage device times (ms) for
         pressure ``p`` (bar), temperature ``T`` (K) and mole fraction
-        (radis/gpu/gpu.py:302-370).  ``l`` and ``slit_FWHM`` are accepted
-        and ignored, as in the reference."""
+        (radis/gpu/gpu.py:302-370).  ``slit_FWHM`` is accepted and ignored,
+        as in the reference; so is ``l`` unless ``derived`` is set: then the
+        last GPU pass also writes absorbance, emissivity, transmittance and
+        radiance for a path of ``l`` cm (radis/lbl/factory.py:1289-1297),
+        left as float64 arrays in ``self.derived``."""
         req = self._request(p, T, mole_fraction)
         t0 = time.perf_counter()
         if self._batch is None:
@@ -101,6 +112,10 @@ class gpuApp:
             self._out = self._batch.alloc_outputs()
         else:
             self._batch.set_request(0, req)
+        want = float(l) if derived else None
+        if want != getattr(self, "_derived_l", None):
+            self._batch.set_derived(0, want or 0.0, which=SpectrumBatch.DERIVED if derived else ())
+            self._derived_l = want
         self._batch.run(self._out)
         inf = self._batch.info(0)
         if inf.status:
@@ -110,6 +125,7 @@ class gpuApp:
         if inf.scale_log2:
             with np.errstate(over="ignore"):
                 out = np.ldexp(out.astype(np.float64), int(inf.scale_log2)).astype(np.float32)
+        self.derived = self._batch.derived_results(0) if derived else {}
         times = self._batch.stage_ms()
         times["total"] = (time.perf_counter() - t0) * 1e3
         if verbose == 1:

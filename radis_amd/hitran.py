@@ -2,7 +2,8 @@
 
 Fixed-width fields (HITRAN 2004 format, as read by RADIS's ``hit2df``,
 radis/api/hitranapi.py): molecule (2), isotopologue (1), nu (12), S (10),
-A (10), gamma_air (5), gamma_self (5), E'' (10), n_air (4), delta_air (8).
+A (10), gamma_air (5), gamma_self (5), E'' (10), n_air (4), delta_air (8),
+and the upper-state weight g' (7) at columns 147-153.
 """
 from __future__ import annotations
 
@@ -26,18 +27,24 @@ _FIELDS = [
     ("n_air", 55, 59, float),
     ("delta", 59, 67, float),
 ]
+# upper-state statistical weight g' (HITRAN 2004 columns 147-153), read when present
+_GU = ("gu", 146, 153, float)
 
 
 def read_par(path):
     """Columns of a .par file as a dict of numpy arrays (file order)."""
     cols = {k: [] for k, *_ in _FIELDS}
+    gu = []
     with open(path) as f:
         for line in f:
-            if len(line.rstrip("\n")) < 67:
+            rec = line.rstrip("\n")
+            if len(rec) < 67:
                 continue
             for k, a, b, conv in _FIELDS:
-                cols[k].append(conv(line[a:b]))
+                cols[k].append(conv(rec[a:b]))
+            gu.append(float(rec[_GU[1]:_GU[2]]) if len(rec) >= _GU[2] and rec[_GU[1]:_GU[2]].strip() else np.nan)
     out = {k: np.asarray(v) for k, v in cols.items()}
+    out["gu"] = np.asarray(gu, dtype=np.float64)
     out["iso"] = out["iso"].astype(np.int64)
     return out
 

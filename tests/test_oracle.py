@@ -209,3 +209,51 @@ def test_hitran_fixture_end_to_end_runs_with_shift_and_isotopes():
     assert lines.iso.max() == 3 and np.any(lines.delta != 0)
     a, det = o.eq_abscoeff(lines.as_dict(), 2000.0, 2300.0, 0.01, 1500.0, 2.0, 0.5, return_details=True)
     assert det["i1"] - det["i0"] == len(lines) and np.all(np.isfinite(a)) and a.max() > 0
+
+
+# --- derived quantities: lbl/factory.py:1289-1297, phys/blackbody.py:35-74 ------
+def test_planck_reference_known_answer_and_wien():
+    """test/phys/test_blackbody.py:82-138: at 2200 K and eps 0.36 the maximum
+    is 75.98736024707178 W/sr/nm/m2 (= 10x mW/cm2/sr/nm) at 2898/T um."""
+    lam = np.arange(1000.0, 1600.0, 0.01)
+    I = 0.36 * o.planck(lam, 2200.0) * 10.0
+    assert np.isclose(I.max(), 75.98736024707178)
+    assert np.isclose(2898 / 2200 * 1e3, lam[I.argmax()], rtol=1e-4)
+
+
+@pytest.mark.parametrize("T", [300.0, 1500.0, 5800.0])
+def test_planck_stefan_boltzmann_and_rayleigh_jeans(T):
+    """pi * integral of B over wavelength = sigma T^4 (CODATA sigma), and
+    the long-wavelength limit 2 c k T / lambda^4."""
+    lam = np.geomspace(10.0, 1e7, 400_001)  # nm
+    B = o.planck(lam, T) * 1e10  # W/sr/m3
+    lm = lam * 1e-9
+    P = np.pi * np.sum(0.5 * (B[1:] + B[:-1]) * np.diff(lm))
+    assert np.isclose(P, 5.670374419e-8 * T**4, rtol=2e-4)
+    lam_far = 1e9  # nm
+    rj = 2 * o.c * o.k_b * T / (lam_far * 1e-9) ** 4 * 1e-10
+    assert np.isclose(o.planck(lam_far, T), rj, rtol=1e-5)
+
+
+def test_cm2nm_and_radiance_through_wavenumber():
+    """cm2nm is 1e7/nu; radiance = emissivity * Planck at that wavelength."""
+    w = np.array([500.0, 2000.0, 2300.123, 10000.0])
+    np.testing.assert_allclose(o.cm2nm(w), 1e7 / w, rtol=1e-15)
+    eps = np.array([0.0, 0.1, 0.5, 1.0])
+    d = o.derived_quantities(w, -np.log1p(-eps), 1.0, 1200.0)
+    np.testing.assert_allclose(d["emissivity"], eps, atol=1e-15)
+    np.testing.assert_allclose(d["radiance"], eps * o.planck(1e7 / w, 1200.0), rtol=1e-13)
+
+
+def test_derived_quantities_limits():
+    """transmittance = exp(-k L); emissivity ~ k L for a thin path and -> 1
+    for a thick one; absorbance linear in the path length."""
+    k = np.array([0.0, 1e-12, 1e-6, 0.3, 5.0, 800.0])
+    w = np.full(k.size, 2100.0)
+    d1 = o.derived_quantities(w, k, 2.0, 1000.0)
+    d3 = o.derived_quantities(w, k, 6.0, 1000.0)
+    np.testing.assert_allclose(d1["transmittance"], np.exp(-2.0 * k), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(d3["absorbance"], 3 * d1["absorbance"], rtol=1e-15)
+    assert d1["emissivity"][1] == pytest.approx(2e-12, rel=1e-9)
+    assert d1["emissivity"][-1] == 1.0 and d1["transmittance"][-1] == 0.0
+    assert d1["emissivity"][0] == 0.0 and d1["radiance"][0] == 0.0
