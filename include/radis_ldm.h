@@ -89,7 +89,10 @@ typedef struct {
                                  fractions always in FP64 (default: FP32 with an exact
                                  FP64 recomputation next to axis nodes);
                                  RADIS_LDM_GENERIC_FFT: run-time-sized FFT passes even
-                                 where compile-time-sized ones exist                     */
+                                 where compile-time-sized ones exist;
+                                 RADIS_LDM_MIN_RMS: min-RMS LDM weights
+                                 (optimization="min-RMS", lbl/broadening.py:2265-2303)
+                                 instead of the simple ones (2305-2318)                  */
     int32_t fft_n1;           /* 0 = planner; else the four-step split L = fft_n1 * n2
                                  (16 | fft_n1 <= 1024, 7-smooth cofactor)                */
     int32_t tile_w;           /* 0 = planner; else the fill tile width (power of 2)      */
@@ -97,6 +100,7 @@ typedef struct {
 
 #define RADIS_LDM_FP64_WIDTHS 1
 #define RADIS_LDM_GENERIC_FFT 2
+#define RADIS_LDM_MIN_RMS 4
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {

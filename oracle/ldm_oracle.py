@@ -25,6 +25,7 @@ FT templates+folding   lbl/broadening.py:1895-1926, ``voigt_FT`` 828-853,
                        ``gaussian_FT`` 252, ``lorentzian_FT`` 604
 indices                lbl/broadening.py:2151-2154 (``_get_indices``), 2253-2263
 weights (simple)       lbl/broadening.py:2305-2318
+weights (min-RMS)      lbl/broadening.py:2265-2303
 scatter                lbl/broadening.py:2340-2346, 2478-2485;
                        misc/arrays.py:472-502 (``numpy_add_at``)
 FFT apply              lbl/broadening.py:2523-2530
@@ -190,10 +191,32 @@ def interp_position_scalar(x, xp):
     return slope * (x - xp[j]) + j
 
 
-def ldm_scatter(L, NG, NL, ki0, ki1, tvi, li0, li1, tGi, mi0, mi1, tLi, S):
-    """Distribute line intensities on the (L, N_G, N_L) float64 LDM;
-    lbl/broadening.py:2305-2318 (simple weights), 2340-2346, 2478-2485."""
-    avi, aGi, aLi = tvi, tGi, tLi
+def min_rms_weights(tvi, tGi, tLi, wG_dat, wL_dat, wstep, dxG, dxL):
+    """``optimization='min-RMS'`` fractions (avi, aGi, aLi) from the simple
+    ones and the FWHM; lbl/broadening.py:2265-2303."""
+    dxvGi = wstep / wG_dat
+    C1_GG = ((6 * np.pi - 16) / (15 * np.pi - 32)) ** (1 / 1.50)
+    C1_LG = ((6 * np.pi - 16) / 3 * (np.log(2) / (2 * np.pi)) ** 0.5) ** (1 / 2.25)
+    C2_GG = (2 * np.log(2) / 15) ** (1 / 1.50)
+    C2_LG = ((2 * np.log(2)) ** 2 / 15) ** (1 / 2.25)
+    alpha_i = wL_dat / wG_dat
+    R_Gv = 8 * np.log(2)
+    R_GG = 2 - 1 / (C1_GG + C2_GG * alpha_i ** (2 / 1.50)) ** 1.50
+    R_GL = -2 * np.log(2) * alpha_i**2
+    R_LL = 1
+    R_LG = 1 / (C1_LG * alpha_i ** (1 / 2.25) + C2_LG * alpha_i ** (4 / 2.25)) ** 2.25
+    avi = tvi
+    aGi = tGi + (R_Gv * tvi * (tvi - 1) * dxvGi**2 + R_GG * tGi * (tGi - 1) * dxG**2
+                 + R_GL * tLi * (tLi - 1) * dxL**2) / (2 * dxG)
+    aLi = tLi + (R_LG * tGi * (tGi - 1) * dxG**2 + R_LL * tLi * (tLi - 1) * dxL**2) / (2 * dxL)
+    return avi, aGi, aLi
+
+
+def ldm_scatter(L, NG, NL, ki0, ki1, avi, li0, li1, aGi, mi0, mi1, aLi, S):
+    """Distribute line intensities on the (L, N_G, N_L) float64 LDM with
+    the fractions (avi, aGi, aLi) along the three axes (the simple weights
+    are the interpolation fractions themselves); lbl/broadening.py:2305-2318,
+    2340-2346, 2478-2485."""
     awV00 = (1 - aGi) * (1 - aLi)
     awV01 = (1 - aGi) * aLi
     awV10 = aGi * (1 - aLi)
@@ -287,10 +310,12 @@ def eq_abscoeff(
     folding_thresh=FOLDING_THRESH_DEFAULT,
     return_details=False,
     fft_workers=None,
+    optimization="simple",
 ):
     """End-to-end abscoeff (cm-1) on the calculation grid, the way
     ``SpectrumFactory.eq_spectrum`` computes it with
-    ``broadening_method='fft', optimization='simple'`` (lbl/factory.py:745-918).
+    ``broadening_method='fft'`` and ``optimization`` ``'simple'`` or
+    ``'min-RMS'`` (lbl/factory.py:745-918).
 
     ``lines['nu0']`` must be sorted; only lines with wmin <= nu0 <= wmax are
     used (lbl/loader.py:2707).  ``L`` defaults to ``2*N_v``."""
@@ -314,14 +339,20 @@ def eq_abscoeff(
     ki0, ki1, tvi = get_indices(shiftwav, w)
     li0, li1, tGi = get_indices(np.log(wG_dat), np.log(wG))
     mi0, mi1, tLi = get_indices(np.log(wL_dat), np.log(wL))
-    LDM = ldm_scatter(L, wG.size, wL.size, ki0, ki1, tvi, li0, li1, tGi, mi0, mi1, tLi, S)
+    if optimization == "min-RMS":
+        avi, aGi, aLi = min_rms_weights(tvi, tGi, tLi, wG_dat, wL_dat, wstep, dxG, dxL)
+    elif optimization == "simple":
+        avi, aGi, aLi = tvi, tGi, tLi
+    else:
+        raise ValueError(f"optimization {optimization!r}")
+    LDM = ldm_scatter(L, wG.size, wL.size, ki0, ki1, avi, li0, li1, aGi, mi0, mi1, aLi, S)
     templates = lineshape_templates(L, wstep, wG, wL, folding_thresh)
     sumoflines = ldm_apply_fft(LDM, templates, Nv, wstep, fft_workers)
     abscoeff = sumoflines * number_density(mole_fraction, pressure_bar, Tgas)
     if return_details:
         return abscoeff, dict(
             i0=i0, i1=i1, Nv=Nv, L=L, NG=wG.size, NL=wL.size, wG=wG, wL=wL,
-            ki0=ki0, tvi=tvi, li0=li0, tGi=tGi, mi0=mi0, tLi=tLi, S=S,
+            ki0=ki0, tvi=tvi, li0=li0, tGi=tGi, mi0=mi0, tLi=tLi, S=S, aGi=aGi, aLi=aLi,
             wG_dat=wG_dat, wL_dat=wL_dat,
         )
     return abscoeff

@@ -81,6 +81,7 @@ class Request(ctypes.Structure):
 
 FP64_WIDTHS = 1
 GENERIC_FFT = 2
+MIN_RMS = 4
 
 
 class RequestInfo(ctypes.Structure):

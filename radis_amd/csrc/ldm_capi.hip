@@ -473,6 +473,18 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.pa_self_f = (float)(J.p_atm * J.x_self);
   for (int s = 0; s < LDM_MAX_ISO; ++s) J.lgc_f[s] = J.dopp[s] > 0 ? (float)std::log(2 * J.dopp[s] / 2.99792458e10) : 0.0f;
   J.fp64_widths = (r.flags & RADIS_LDM_FP64_WIDTHS) ? 1 : 0;
+  J.min_rms = (r.flags & RADIS_LDM_MIN_RMS) ? 1 : 0;
+  {
+    // constants of the min-RMS corrections, radis/lbl/broadening.py:2271-2276
+    const double pi = 3.14159265358979323846, ln2 = std::log(2.0);
+    J.mr_dv = (float)r.wstep;
+    J.mr_dxG = (float)r.dxG;
+    J.mr_dxL = (float)r.dxL;
+    J.mr_c1gg = (float)std::pow((6 * pi - 16) / (15 * pi - 32), 1 / 1.50);
+    J.mr_c1lg = (float)std::pow((6 * pi - 16) / 3 * std::pow(ln2 / (2 * pi), 0.5), 1 / 2.25);
+    J.mr_c2gg = (float)std::pow(2 * ln2 / 15, 1 / 1.50);
+    J.mr_c2lg = (float)std::pow((2 * ln2) * (2 * ln2) / 15, 1 / 2.25);
+  }
   J.c_stim_T = (float)(-HC_K / r.Tgas);
   J.c_stim_R = (float)(-HC_K / r.Tref);
   J.out_scale = (float)(1.0 / ((double)R.plan.L * r.wstep));

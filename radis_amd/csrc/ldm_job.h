@@ -64,6 +64,9 @@ struct LdmJob {
   float laG0_f, inv_dxG_f;       // first G-axis node (log) and 1/dxG, FP32 (set by k_finalize)
   float laL0_f, inv_dxL_f;       // same for the L axis (set by k_finalize)
   int fp64_widths;               // 1: log-width positions always in FP64
+  int min_rms;                   // 1: min-RMS weights
+  float mr_dv, mr_dxG, mr_dxL;   // wstep, dxG, dxL of the min-RMS corrections
+  float mr_c1gg, mr_c2gg, mr_c1lg, mr_c2lg;
   float out_scale;               // 1/(L*wstep)
   float inv_Lws;                 // 1/(L*wstep) (frequency step of the FT grid)
   float half_inv_ws;             // 1/(2*wstep)

@@ -490,14 +490,37 @@ struct LineConst {
   double c_boltz, i_scale;
   float c_stim_T, c_stim_R, log_r_f, pa_air_f, pa_self_f;
   float laG0_f, inv_dxG_f, laL0_f, inv_dxL_f;
-  int nv, fp64_widths, has_delta;
+  int nv, fp64_widths, has_delta, min_rms;
+  const LdmJob* job;
   __device__ explicit LineConst(const LdmJob& J, const LdmDb& db)
       : wmin(J.wmin), delta_w(J.delta_w), inv_delta_w(J.inv_delta_w), p_atm(J.p_atm), x_air(J.x_air),
         x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
         c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
         laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
-        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr) {}
+        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), job(&J) {}
 };
+
+// min-RMS weights, radis/lbl/broadening.py:2265-2303: the Gaussian and
+// Lorentzian fractions get second-order corrections in the three fractions,
+// with coefficients from the width ratio alpha = wL/wG (FWHM) and
+// dv/wG.  FP32: the corrections feed the FP32 weights either way.
+__device__ __forceinline__ void min_rms_weights(const LdmJob& J, float log_wg, float wl, LineTerms& t) {
+  const float wg = __expf(log_wg);
+  const float dxv = J.mr_dv / wg;
+  const float la = __logf(wl) - log_wg;  // log(alpha)
+  const float dxG = J.mr_dxG, dxL = J.mr_dxL;
+  const float ln2 = 0.69314718055994531f;
+  const float r_gv = 8.0f * ln2;
+  const float r_gg = 2.0f - 1.0f / __powf(J.mr_c1gg + J.mr_c2gg * __expf(la * (2.0f / 1.50f)), 1.50f);
+  const float r_gl = -2.0f * ln2 * __expf(2.0f * la);
+  const float r_lg = 1.0f / __powf(J.mr_c1lg * __expf(la * (1.0f / 2.25f)) + J.mr_c2lg * __expf(la * (4.0f / 2.25f)),
+                                   2.25f);
+  const float qv = t.tv * (t.tv - 1.0f), qg = t.tg * (t.tg - 1.0f), ql = t.tl * (t.tl - 1.0f);
+  const float ag = t.tg + (r_gv * qv * dxv * dxv + r_gg * qg * dxG * dxG + r_gl * ql * dxL * dxL) / (2.0f * dxG);
+  const float al = t.tl + (r_lg * qg * dxG * dxG + ql * dxL * dxL) / (2.0f * dxL);
+  t.tg = ag;
+  t.tl = al;
+}
 
 // Per-workgroup LDS tables of the line computation.
 struct LineTables {
@@ -573,6 +596,7 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   }
   if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
   if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
+  if (C.min_rms) min_rms_weights(*C.job, __logf(nuf) + iv.lgc, wl_f, t);
   const double s64 = (double)r.S * iv.q * C.i_scale;
   // (1 - exp(-c2 nu/T)) / (1 - exp(-c2 nu/Tref)), radis/lbl/base.py:2664-2674:
   // for |x| >= 0.1 the difference 1 - e^x loses under 4 bits, so the fast
@@ -588,8 +612,8 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 // ---------------------------------------------------------------------------
 // k_fill: one workgroup per (request, tile of W grid points).  Every line
 // whose 2x2x2 stencil touches the tile gets its LineTerms, then adds the
-// simple weights (radis/lbl/broadening.py:2305-2318) of its corners that
-// fall in the tile into LDS with ds_add_f32.  The tile is written to HBM
+// weights (radis/lbl/broadening.py:2305-2318) of its corners that fall in
+// the tile into the LDS tile with FP64 atomics.  The tile is written to HBM
 // once, so the LDM needs no clearing and no global atomics.  Row index
 // r = l*N_L + m; LDS row stride W+2 doubles (16-byte aligned rows, rows on
 // different banks).  The tile accumulates in FP64 with ds_add_f64: on gfx950

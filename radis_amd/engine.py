@@ -184,13 +184,17 @@ class LineDatabase:
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
                  fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0,
-                 generic_fft=False):
+                 generic_fft=False, optimization="simple"):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
     (lo, hi) overrides the line selection [wmin, wmax]; ``fft_n1``/``tile_w``
     fix the FFT split and the fill tile width (default: the autotuner's cache,
-    else the planner); ``generic_fft`` selects the run-time-sized FFT passes."""
+    else the planner); ``generic_fft`` selects the run-time-sized FFT passes;
+    ``optimization`` is the LDM weighting, ``"simple"`` or ``"min-RMS"``
+    (radis/lbl/broadening.py:2265-2318)."""
+    if optimization not in ("simple", "min-RMS"):
+        raise ValueError(f"optimization must be 'simple' or 'min-RMS', not {optimization!r}")
     r = _native.Request()
     r.wmin, r.wmax, r.wstep = float(wmin), float(wmax), float(wstep)
     r.Tgas, r.pressure_bar, r.mole_fraction = float(Tgas), float(pressure_bar), float(mole_fraction)
@@ -200,7 +204,8 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
         r.q_ratio[s] = float(q[s]) if s < q.size else 1.0
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
-    r.flags = (_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
+    r.flags = ((_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
+               | (_native.MIN_RMS if optimization == "min-RMS" else 0))
     r.fft_n1, r.tile_w = int(fft_n1 or 0), int(tile_w or 0)
     if line_window is not None:
         r.line_lo, r.line_hi = float(line_window[0]), float(line_window[1])

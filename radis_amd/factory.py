@@ -91,7 +91,7 @@ class SpectrumFactory:
 
     def __init__(self, wavenum_min=None, wavenum_max=None, wstep=0.01, Tref=296, pressure=1.01325,
                  mole_fraction=1, path_length=1, dxG=DXG_DEFAULT, dxL=DXL_DEFAULT,
-                 folding_thresh=FOLDING_THRESH_DEFAULT, lines=None, verbose=0, **kwargs):
+                 folding_thresh=FOLDING_THRESH_DEFAULT, optimization="simple", lines=None, verbose=0, **kwargs):
         if wavenum_min is None or wavenum_max is None:
             raise ValueError("wavenum_min and wavenum_max (cm-1) are required")
         if kwargs.get("neighbour_lines", 0):
@@ -100,6 +100,9 @@ class SpectrumFactory:
         self.Tref = float(Tref)
         self.pressure, self.mole_fraction, self.path_length = pressure, mole_fraction, path_length
         self.dxG, self.dxL, self.folding_thresh = dxG, dxL, folding_thresh
+        if optimization not in ("simple", "min-RMS"):
+            raise NotImplementedError(f"optimization={optimization!r} (this package: 'simple' or 'min-RMS')")
+        self.optimization = optimization
         self.verbose = verbose
         self.lines = None
         if lines is not None:
@@ -154,6 +157,7 @@ class SpectrumFactory:
             mole_fraction=self.mole_fraction, path_length=self.path_length, wstep=self.wstep,
             wavenum_min=self.wavenum_min, wavenum_max=self.wavenum_max, dxG=self.dxG, dxL=self.dxL,
             NwL=NwL, NwG=NwG, lines_calculated=int(cols["nu0"].size), thermal_equilibrium=True,
+            optimization=self.optimization, broadening_method="fft",
             diluents={"air": 1 - self.mole_fraction}, gpu_backend="hip", spectral_points=int(w.size),
             add_at_used="gpu-backend", woutrange=(0, int(w.size)), calculation_time=time.perf_counter() - t0,
         )
@@ -184,7 +188,7 @@ class _LineApp:
     def _req(self, p, T, x):
         sf = self.sf
         return engine.make_request(self._db, sf.wavenum_min, sf.wavenum_max, sf.wstep, T, p, x, sf.dxG, sf.dxL,
-                                   sf.Tref, sf.folding_thresh)
+                                   sf.Tref, sf.folding_thresh, optimization=sf.optimization)
 
     def iterate(self, p, T, x, path_length=None):
         """abscoeff (cm-1) at (p bar, T K, x); with ``path_length`` (cm) the

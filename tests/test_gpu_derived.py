@@ -203,3 +203,18 @@ def test_calc_spectrum_gpu_mode():
     ref = oracle.derived_quantities(w, a, 5.0, 1200.0)
     assert oracle.rel_peak_error(rad, ref["radiance"]) <= 2e-6
     assert s.gpu_app is None
+
+
+def test_factory_min_rms_option():
+    lines = hitran.line_list(HITRAN)
+    sf = factory.SpectrumFactory(wavenum_min=2150.0, wavenum_max=2200.0, wstep=0.001, pressure=0.05,
+                                 mole_fraction=0.1, optimization="min-RMS", lines=lines)
+    s = sf.eq_spectrum_gpu(Tgas=2500.0)
+    L = int(s.gpu_app._batch.info(0).fft_len)
+    ref = oracle.eq_abscoeff(lines.as_dict(), 2150.0, 2200.0, 0.001, 2500.0, 0.05, 0.1, engine.DXG_DEFAULT,
+                             engine.DXL_DEFAULT, L=L, optimization="min-RMS")
+    assert oracle.rel_peak_error(s.get("abscoeff")[1], ref) <= REL_TOL
+    assert s.conditions["optimization"] == "min-RMS"
+    s.exit_gpu()
+    with pytest.raises(NotImplementedError):
+        factory.SpectrumFactory(wavenum_min=2150.0, wavenum_max=2200.0, optimization=None, lines=lines)

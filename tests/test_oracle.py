@@ -257,3 +257,57 @@ def test_derived_quantities_limits():
     assert d1["emissivity"][1] == pytest.approx(2e-12, rel=1e-9)
     assert d1["emissivity"][-1] == 1.0 and d1["transmittance"][-1] == 0.0
     assert d1["emissivity"][0] == 0.0 and d1["radiance"][0] == 0.0
+
+
+# --- min-RMS weights: lbl/broadening.py:2265-2303 --------------------------------
+def test_min_rms_weights_reduce_to_simple_on_nodes():
+    """Every correction term carries t(t-1) of some axis: on grid and axis
+    nodes the min-RMS fractions are the simple ones."""
+    t = np.array([0.0, 1.0, 0.0, 1.0])
+    u = np.array([0.0, 0.0, 1.0, 1.0])
+    wG, wL = np.array([0.004, 0.01, 0.03, 0.1]), np.array([0.1, 0.002, 0.05, 0.3])
+    av, aG, aL = o.min_rms_weights(t, u, t[::-1], wG, wL, 0.002, 0.14, 0.2)
+    np.testing.assert_array_equal(av, t)
+    np.testing.assert_array_equal(aG, u)
+    np.testing.assert_array_equal(aL, t[::-1])
+
+
+def _rms_ratio(regime, n_cases=8):
+    """Median over random 3-line spectra of RMS(min-RMS - Voigt) /
+    RMS(simple - Voigt), the exact Voigt profile from scipy."""
+    rng = np.random.default_rng(3)
+    if regime == "lorentz":
+        wstep, T_rng, p_rng = 0.002, (300.0, 3000.0), (0.05, 2.0)
+    else:
+        wstep, T_rng, p_rng = 0.0005, (1500.0, 3000.0), (0.002, 0.05)
+    w = o.wavenumber_grid(2000.0, 2010.0, wstep)
+    ratios = []
+    for _ in range(n_cases):
+        nu = np.sort(rng.uniform(2004.0, 2006.0, 3))
+        lines = dict(nu0=nu, S_ref=np.full(3, 1e-20), El=np.zeros(3), g_air=rng.uniform(0.01, 0.1, 3),
+                     g_self=rng.uniform(0.01, 0.1, 3), n_air=np.full(3, 0.7), molar_mass=np.array([0.0, 28.0]),
+                     Q=lambda i, T: 1.0)
+        T, p = rng.uniform(*T_rng), rng.uniform(*p_rng)
+        err = {}
+        for opt in ("simple", "min-RMS"):
+            a, det = o.eq_abscoeff(lines, 2000.0, 2010.0, wstep, T, p, 0.1, 0.2, 0.2, return_details=True,
+                                   optimization=opt)
+            ex = sum(det["S"][j] * voigt_profile(w - nu[j], det["wG_dat"][j] / (2 * np.sqrt(2 * np.log(2))),
+                                                 det["wL_dat"][j] / 2) for j in range(3))
+            ex = ex * o.number_density(0.1, p, T)
+            err[opt] = np.sqrt(np.mean((a - ex) ** 2))
+        ratios.append(err["min-RMS"] / err["simple"])
+    return float(np.median(ratios)), float(np.max(ratios))
+
+
+def test_min_rms_weights_lower_the_error_against_voigt():
+    """The min-RMS weights are built to minimise the RMS error of the LDM
+    lineshape (van den Bekerom & Pannier 2021).  With dxG = dxL = 0.2 the
+    median error falls ~3x for pressure-broadened lines and ~9x for
+    Doppler-broadened ones.  Each single-term mistake tried (a term dropped,
+    R_GL's sign flipped, R_GG set to 1, dv/wG inverted) moves one of the two
+    medians past its bound."""
+    med, worst = _rms_ratio("lorentz")
+    assert med <= 0.5 and worst < 1.0, (med, worst)
+    med, worst = _rms_ratio("doppler")
+    assert med <= 0.14 and worst < 0.3, (med, worst)
