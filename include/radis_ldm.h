@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RADIS_LDM_ABI_VERSION 1
+#define RADIS_LDM_ABI_VERSION 2
 
 enum {
     RADIS_LDM_OK = 0,
@@ -64,6 +64,10 @@ typedef struct {
     int64_t n;                /* number of lines                                        */
     const double *molar_mass; /* [n_iso_slots] g/mol, indexed by iso (slot 0 unused)    */
     int32_t n_iso_slots;      /* <= RADIS_LDM_MAX_ISO                                   */
+    const float *Evib_l;      /* [n] or NULL: vibrational energy of the lower state, cm-1
+                                 (0 <= Evib_l <= El); needed by RADIS_LDM_NONEQ requests */
+    const float *Evib_u;      /* [n] or NULL: vibrational energy of the upper state, cm-1
+                                 (0 <= Evib_u <= El + nu0); with Evib_l                   */
 } radis_ldm_lines;
 
 #define RADIS_LDM_MAX_ISO 16
@@ -96,11 +100,19 @@ typedef struct {
     int32_t fft_n1;           /* 0 = planner; else the four-step split L = fft_n1 * n2
                                  (16 | fft_n1 <= 1024, 7-smooth cofactor)                */
     int32_t tile_w;           /* 0 = planner; else the fill tile width (power of 2)      */
+    double Tvib, Trot;        /* RADIS_LDM_NONEQ: vibrational and rotational temperatures
+                                 (K) of the level populations; Tgas is then the
+                                 translational temperature of the widths, and q_ratio
+                                 holds Q(Tref)/Q_neq(Tvib, Trot) per iso slot             */
 } radis_ldm_request;
 
 #define RADIS_LDM_FP64_WIDTHS 1
 #define RADIS_LDM_GENERIC_FFT 2
 #define RADIS_LDM_MIN_RMS 4
+/* Two-temperature line strengths (Boltzmann vibrational and rotational
+ * distributions, radis/lbl/base.py:3479-3530 populations and 3937-3947
+ * calc_linestrength_noneq); the line list must carry Evib_l and Evib_u. */
+#define RADIS_LDM_NONEQ 8
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {

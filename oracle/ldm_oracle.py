@@ -16,6 +16,8 @@ step                   reference
 grid                   lbl/factory.py:2590-2617 (``arange(wmin, wmax+wstep, wstep)``)
 line selection         lbl/loader.py:2707 (``wav >= wmin & wav <= wmax``)
 S(T)                   lbl/base.py:2664-2674 (``calc_linestrength_eq``)
+S(Tvib, Trot)          lbl/base.py:3479-3530 (Boltzmann populations),
+                       3937-3947 (``calc_linestrength_noneq``), S0 2437-2441
 pressure shift         lbl/base.py:2351-2360 (``calc_lineshift``)
 Doppler HWHM           lbl/broadening.py:170-172 (``doppler_broadening_HWHM``)
 Lorentzian HWHM        lbl/broadening.py:341-383 (``pressure_broadening_HWHM``),
@@ -84,6 +86,33 @@ def linestrength_eq(S_ref, El, nu0, Tgas, Tref, Qref_over_Qgas):
         * (1 - np.exp(-hc_k * nu0 / Tgas))
         / (1 - np.exp(-hc_k * nu0 / Tref))
     )
+
+
+def s0_einstein(A, gu, nu0, Ia):
+    """Intensity without the population term, ``Ia gu A / (8 pi c nu0^2)``;
+    lbl/base.py:2437-2441."""
+    return Ia * gu * A / (8 * np.pi * c_CGS * nu0**2)
+
+
+def populations_noneq(gvibl, grotl, Evibl, Erotl, gvibu, grotu, Evibu, Erotu, Tvib, Trot, Qneq):
+    """Upper and lower state populations (nu, nl) for Boltzmann vibrational
+    and rotational distributions without electronic states;
+    lbl/base.py:3479-3530."""
+    nu_vib_x_Qvib = gvibu * np.exp(-hc_k * Evibu / Tvib)
+    nl_vib_x_Qvib = gvibl * np.exp(-hc_k * Evibl / Tvib)
+    nu_rot_x_Qrot = grotu * np.exp(-Erotu * hc_k / Trot)
+    nl_rot_x_Qrot = grotl * np.exp(-Erotl * hc_k / Trot)
+    nu = nu_vib_x_Qvib * nu_rot_x_Qrot / Qneq
+    nl = nl_vib_x_Qvib * nl_rot_x_Qrot / Qneq
+    return nu, nl
+
+
+def linestrength_noneq(S0, gl, gu, nu, nl):
+    """Non-LTE linestrength from the populations; lbl/base.py:3937-3947."""
+    line_strength = S0 / gl
+    line_strength = line_strength * nl
+    line_strength = line_strength * (1 - gl / gu * nu / nl)
+    return line_strength
 
 
 def lineshift(nu0, delta_air, pressure_bar):
@@ -262,12 +291,42 @@ def number_density(mole_fraction, pressure_bar, Tgas):
     return mole_fraction * ((pressure_bar * 1e5) / (k_b * Tgas)) * 1e-6
 
 
-def line_parameters(lines, Tgas, pressure_bar, mole_fraction, Tref=296.0):
+def _noneq_strength(lines, nu0, iso, Tvib, Trot, Tref):
+    """S(Tvib, Trot) of every line.  The lines give ``Evib_l``, ``Evib_u``
+    and ``Qneq(iso, Tvib, Trot)``, and either ``S0`` (Einstein form) or
+    ``S_ref`` with ``Q``, from which S0 follows as
+    S_ref Q(Tref) / (exp(-c2 El/Tref) (1 - exp(-c2 nu0/Tref))).  Degeneracies:
+    ``gl``, ``gu`` (total) with the vibrational one ``gvibl``/``gvibu``
+    (default 1) and the rotational one the quotient; 1 when absent.  Level
+    energies: E_l = El, E_u = El + nu0, Erot = E - Evib."""
+    f64 = lambda k, d=None: d if lines.get(k) is None else np.asarray(lines[k], dtype=np.float64)  # noqa: E731
+    El = f64("El")
+    isoi = np.ones(nu0.shape, dtype=np.int64) if iso is None else np.asarray(iso, dtype=np.int64)
+    if lines.get("S0") is not None:
+        S0 = f64("S0")
+    else:
+        Qref = np.array([lines["Q"](int(i), Tref) if i > 0 else 1.0 for i in range(int(isoi.max()) + 1)])[isoi]
+        S0 = f64("S_ref") * Qref / (np.exp(-hc_k * El / Tref) * (1 - np.exp(-hc_k * nu0 / Tref)))
+    one = np.ones(nu0.shape)
+    gl, gu = f64("gl", one), f64("gu", one)
+    gvibl, gvibu = f64("gvibl", one), f64("gvibu", one)
+    Evibl, Evibu = f64("Evib_l"), f64("Evib_u")
+    Eu = El + nu0
+    Qneq = np.array([lines["Qneq"](int(i), Tvib, Trot) if i > 0 else 1.0
+                     for i in range(int(isoi.max()) + 1)])[isoi]
+    nu, nl = populations_noneq(gvibl, gl / gvibl, Evibl, El - Evibl, gvibu, gu / gvibu, Evibu, Eu - Evibu,
+                               Tvib, Trot, Qneq)
+    return linestrength_noneq(S0, gl, gu, nu, nl)
+
+
+def line_parameters(lines, Tgas, pressure_bar, mole_fraction, Tref=296.0, Tvib=None, Trot=None):
     """Per-line S(T), shifted centre, Gaussian and Lorentzian FWHM (float64).
 
     ``lines`` is a dict of arrays: ``nu0, S_ref, El, g_air, g_self, n_air``
     and optionally ``n_self``, ``delta``, ``iso``; plus ``molar_mass``
-    (indexable by iso) and ``Q`` (callable ``Q(iso, T)``)."""
+    (indexable by iso) and ``Q`` (callable ``Q(iso, T)``).  With ``Tvib``
+    and ``Trot`` the line strengths are the two-temperature ones
+    (``_noneq_strength``) and ``Tgas`` is the translational temperature."""
     nu0 = np.asarray(lines["nu0"], dtype=np.float64)
     iso = lines.get("iso")
     mm = np.asarray(lines["molar_mass"], dtype=np.float64)
@@ -282,7 +341,10 @@ def line_parameters(lines, Tgas, pressure_bar, mole_fraction, Tref=296.0):
         )
         qratio = qtab[iso]
     f64 = lambda k: None if lines.get(k) is None else np.asarray(lines[k], dtype=np.float64)
-    S = linestrength_eq(f64("S_ref"), f64("El"), nu0, Tgas, Tref, qratio)
+    if Tvib is not None:
+        S = _noneq_strength(lines, nu0, iso, Tvib, Trot, Tref)
+    else:
+        S = linestrength_eq(f64("S_ref"), f64("El"), nu0, Tgas, Tref, qratio)
     shiftwav = lineshift(nu0, f64("delta"), pressure_bar)
     wG_dat = doppler_hwhm(nu0, molar_mass, Tgas) * 2.000
     wL_dat = (
@@ -311,11 +373,17 @@ def eq_abscoeff(
     return_details=False,
     fft_workers=None,
     optimization="simple",
+    Tvib=None,
+    Trot=None,
 ):
     """End-to-end abscoeff (cm-1) on the calculation grid, the way
     ``SpectrumFactory.eq_spectrum`` computes it with
     ``broadening_method='fft'`` and ``optimization`` ``'simple'`` or
-    ``'min-RMS'`` (lbl/factory.py:745-918).
+    ``'min-RMS'`` (lbl/factory.py:745-918); with ``Tvib`` and ``Trot`` the
+    line strengths are the two-temperature ones of ``non_eq_spectrum``
+    (lbl/factory.py:1525-1952, Ttrans = Trot by default at 1685-1687, calling
+    lbl/base.py:3360 and 3879) and ``Tgas``
+    is the translational temperature.
 
     ``lines['nu0']`` must be sorted; only lines with wmin <= nu0 <= wmax are
     used (lbl/loader.py:2707).  ``L`` defaults to ``2*N_v``."""
@@ -333,7 +401,7 @@ def eq_abscoeff(
         if return_details:
             return out, dict(i0=i0, i1=i1, Nv=Nv, L=L, NG=0, NL=0)
         return out
-    S, shiftwav, wG_dat, wL_dat = line_parameters(sub, Tgas, pressure_bar, mole_fraction, Tref)
+    S, shiftwav, wG_dat, wL_dat = line_parameters(sub, Tgas, pressure_bar, mole_fraction, Tref, Tvib, Trot)
     wL = init_w_axis(wL_dat, dxL)
     wG = init_w_axis(wG_dat, dxG)
     ki0, ki1, tvi = get_indices(shiftwav, w)

@@ -53,6 +53,8 @@ class Lines(ctypes.Structure):
         ("n", c_int64),
         ("molar_mass", POINTER(c_double)),
         ("n_iso_slots", c_int32),
+        ("Evib_l", POINTER(c_float)),
+        ("Evib_u", POINTER(c_float)),
     ]
 
 
@@ -76,12 +78,16 @@ class Request(ctypes.Structure):
         ("flags", c_int32),
         ("fft_n1", c_int32),
         ("tile_w", c_int32),
+        ("Tvib", c_double),
+        ("Trot", c_double),
     ]
 
 
 FP64_WIDTHS = 1
 GENERIC_FFT = 2
 MIN_RMS = 4
+NONEQ = 8
+ABI_VERSION = 2
 
 
 class RequestInfo(ctypes.Structure):
@@ -165,6 +171,9 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.radis_ldm_abi_version() != ABI_VERSION:
+        raise NativeLibraryError(f"{LIB_PATH} has ABI {lib.radis_ldm_abi_version()}, this binding {ABI_VERSION}: "
+                                 "rebuild it")
     _lib = lib
     return lib
 

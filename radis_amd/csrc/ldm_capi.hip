@@ -97,7 +97,7 @@ struct radis_ldm_db {
   void* mem = nullptr;
   size_t bytes = 0;
   LdmDb dev{};
-  bool has_iso = false, has_nself = false, has_delta = false;
+  bool has_iso = false, has_nself = false, has_delta = false, has_evib = false;
   double molar_mass[LDM_MAX_ISO] = {0};
   int n_iso_slots = 0;
   unsigned iso_mask = 0;
@@ -105,6 +105,7 @@ struct radis_ldm_db {
   std::vector<double> h_nu0;
   // per-block statistics used for workspace and scale bounds
   std::vector<float> ga_min, ga_max, gs_min, gs_max, na_min, na_max, ns_min, ns_max, e_min, e_max, s_max;
+  std::vector<float> dv_min, dv_max;  // Evib_u - Evib_l (two-temperature bound)
 };
 
 namespace {
@@ -364,6 +365,8 @@ int validate_request(const radis_ldm_request& r) {
   if (!(r.mole_fraction >= 0) || !(r.mole_fraction <= 1)) return fail(RADIS_LDM_EINVAL, "mole_fraction must be in [0, 1]");
   if (!(r.dxG > 0) || !(r.dxL > 0)) return fail(RADIS_LDM_EINVAL, "dxG and dxL must be positive");
   if (!(r.folding_thresh > 0)) return fail(RADIS_LDM_EINVAL, "folding_thresh must be positive");
+  if ((r.flags & RADIS_LDM_NONEQ) && (!(r.Tvib > 0) || !(r.Trot > 0)))
+    return fail(RADIS_LDM_EINVAL, "RADIS_LDM_NONEQ needs positive Tvib and Trot");
   return 0;
 }
 
@@ -376,6 +379,8 @@ double dopp_factor(double T, double M) {
 // conditions, from per-block statistics of the window's lines.
 int bound_request(const radis_ldm_db& db, Req& R) {
   const radis_ldm_request& r = R.r;
+  if ((r.flags & RADIS_LDM_NONEQ) && !db.has_evib)
+    return fail(RADIS_LDM_EINVAL, "RADIS_LDM_NONEQ needs a line list with Evib_l and Evib_u");
   if (R.i1 <= R.i0) {
     R.m_need = 1;
     R.scale_log2 = 0;
@@ -418,16 +423,34 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   // intensity scale: log2 of an upper bound of S(T) * density
   double lb = -1e300;
   if (smax > 0) {
-    const double cb = -HC_K * (1 / r.Tgas - 1 / r.Tref);
+    const bool neq = (r.flags & RADIS_LDM_NONEQ) != 0;
     double qmax = 0;
     for (int s = 0; s < db.n_iso_slots; ++s)
       if (db.iso_mask & (1u << s)) qmax = std::max(qmax, r.q_ratio[s]);
     const double numin = std::max(db.h_nu0[R.i0], 1e-3);
-    const double stim = (r.Tgas >= r.Tref) ? 1.0 : 1.0 / (1.0 - std::exp(-HC_K * numin / r.Tref));
+    double boltz, stim;
+    if (!neq) {
+      const double cb = -HC_K * (1 / r.Tgas - 1 / r.Tref);
+      boltz = std::max(cb * e0, cb * e1);
+      stim = (r.Tgas >= r.Tref) ? 1.0 : 1.0 / (1.0 - std::exp(-HC_K * numin / r.Tref));
+    } else {
+      // exponent -c2(Evl/Tv + (El-Evl)/Tr - El/Tref) is linear in Evl in
+      // [0, El]: extreme at Evl = 0 or El; the stimulated-emission numerator
+      // 1 - exp(-c2(dEv/Tv + (nu0-dEv)/Tr)) is below 1 unless that exponent
+      // is positive (inversion), where it is below exp(-c2 X_min) over the
+      // block's dEv = Evib_u - Evib_l range
+      const double cv = -HC_K * (1 / r.Tvib - 1 / r.Tref), cr = -HC_K * (1 / r.Trot - 1 / r.Tref);
+      boltz = std::max(std::max(cv * e0, cv * e1), std::max(cr * e0, cr * e1));
+      double dv0 = 1e300, dv1 = -1e300;
+      for (int b = b0; b <= b1; ++b) { dv0 = std::min(dv0, (double)db.dv_min[b]); dv1 = std::max(dv1, (double)db.dv_max[b]); }
+      const double k = 1 / r.Tvib - 1 / r.Trot;
+      const double xmin = HC_K * (std::min(dv0 * k, dv1 * k) + numin / r.Trot);
+      const double num = xmin < 0 ? std::exp(-xmin) : 1.0;
+      stim = num / (1.0 - std::exp(-HC_K * numin / r.Tref));
+    }
     const double dens = r.mole_fraction * ((r.pressure_bar * 1e5) / (K_KB * r.Tgas)) * 1e-6;
     if (qmax > 0 && dens > 0)
-      lb = std::log2(smax) + std::log2(qmax) + std::max(cb * e0, cb * e1) / std::log(2.0) + std::log2(stim) +
-           std::log2(dens);
+      lb = std::log2(smax) + std::log2(qmax) + boltz / std::log(2.0) + std::log2(stim) + std::log2(dens);
   }
   R.scale_log2 = (lb > 80 || (lb < -80 && lb > -1e299)) ? (int)std::lround(lb) - 40 : 0;
   return 0;
@@ -474,6 +497,10 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   for (int s = 0; s < LDM_MAX_ISO; ++s) J.lgc_f[s] = J.dopp[s] > 0 ? (float)std::log(2 * J.dopp[s] / 2.99792458e10) : 0.0f;
   J.fp64_widths = (r.flags & RADIS_LDM_FP64_WIDTHS) ? 1 : 0;
   J.min_rms = (r.flags & RADIS_LDM_MIN_RMS) ? 1 : 0;
+  J.noneq = (r.flags & RADIS_LDM_NONEQ) ? 1 : 0;
+  J.c_vib = J.noneq ? -HC_K / r.Tvib : 0.0;
+  J.c_rot = J.noneq ? -HC_K / r.Trot : 0.0;
+  J.c_ref = HC_K / r.Tref;
   {
     // constants of the min-RMS corrections, radis/lbl/broadening.py:2271-2276
     const double pi = 3.14159265358979323846, ln2 = std::log(2.0);
@@ -920,6 +947,11 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   db->has_iso = L->iso != nullptr;
   db->has_nself = L->n_self != nullptr;
   db->has_delta = L->delta != nullptr;
+  if ((L->Evib_l == nullptr) != (L->Evib_u == nullptr)) {
+    delete db;
+    return fail(RADIS_LDM_EINVAL, "Evib_l and Evib_u come together");
+  }
+  db->has_evib = L->Evib_l != nullptr;
   db->n_iso_slots = L->n_iso_slots;
   for (int s = 0; s < L->n_iso_slots; ++s) db->molar_mass[s] = L->molar_mass[s];
   if (db->has_iso) {
@@ -952,12 +984,29 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
     db->e_min[b] = std::min(db->e_min[b], L->El[i]); db->e_max[b] = std::max(db->e_max[b], L->El[i]);
     db->s_max[b] = std::max(db->s_max[b], L->S_ref[i]);
   }
+  if (db->has_evib) {
+    db->dv_min.assign(nb, 3.4e38f);
+    db->dv_max.assign(nb, -3.4e38f);
+    for (int64_t i = 0; i < n; ++i) {
+      const int b = (int)(i / STAT_BLOCK);
+      const float el = L->El[i], evl = L->Evib_l[i], evu = L->Evib_u[i];
+      if (!(evl >= 0 && evl <= el * (1 + 1e-6f) + 1e-3f && evu >= 0 &&
+            evu <= (el + (float)L->nu0[i]) * (1 + 1e-6f) + 1e-3f)) {
+        delete db;
+        return fail(RADIS_LDM_EINVAL, "line %lld: need 0 <= Evib_l <= El and 0 <= Evib_u <= El + nu0", (long long)i);
+      }
+      const float d = evu - evl;
+      db->dv_min[b] = std::min(db->dv_min[b], d);
+      db->dv_max[b] = std::max(db->dv_max[b], d);
+    }
+  }
   // device arrays: one allocation
-  size_t off[10], tot = 0;
-  const size_t sz[10] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
+  size_t off[12], tot = 0;
+  const size_t sz[12] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
                          db->has_nself ? (size_t)n * 4 : 0, db->has_delta ? (size_t)n * 4 : 0,
-                         db->has_iso ? (size_t)n : 0, 8};
-  for (int k = 0; k < 10; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
+                         db->has_iso ? (size_t)n : 0, 8,
+                         db->has_evib ? (size_t)n * 4 : 0, db->has_evib ? (size_t)n * 4 : 0};
+  for (int k = 0; k < 12; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
   if (ctx->budget > 0 && ctx->in_use + (int64_t)tot > ctx->budget) {
     delete db;
     return fail(RADIS_LDM_ENOMEM, "line list of %zu bytes exceeds the device budget", tot);
@@ -973,8 +1022,9 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   char* m = (char*)db->mem;
   // defaults block: a float 0 (missing delta column) and a byte 1 (missing iso column)
   unsigned char dflt[8] = {0, 0, 0, 0, 1, 0, 0, 0};
-  const void* src[10] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso, dflt};
-  for (int k = 0; k < 10; ++k)
+  const void* src[12] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso, dflt,
+                         L->Evib_l, L->Evib_u};
+  for (int k = 0; k < 12; ++k)
     if (sz[k]) {
       hipError_t e = hipMemcpy(m + off[k], src[k], sz[k], hipMemcpyHostToDevice);
       if (e != hipSuccess) {
@@ -992,6 +1042,8 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   db->dev.delta = db->has_delta ? (const float*)(m + off[7]) : nullptr;
   db->dev.iso = db->has_iso ? (const uint8_t*)(m + off[8]) : nullptr;
   db->dev.dflt = (const float*)(m + off[9]);
+  db->dev.evib_l = db->has_evib ? (const float*)(m + off[10]) : nullptr;
+  db->dev.evib_u = db->has_evib ? (const float*)(m + off[11]) : nullptr;
   *out = db;
   return 0;
 }

@@ -41,6 +41,8 @@ struct LdmDb {
   const float* delta;    // may be null
   const uint8_t* iso;    // may be null
   const float* dflt;     // device block {0.0f, then byte 1}: what a missing delta / iso column reads
+  const float* evib_l;   // lower/upper-state vibrational energies (cm-1), may be null
+  const float* evib_u;
 };
 
 struct LdmJob {
@@ -65,6 +67,8 @@ struct LdmJob {
   float laL0_f, inv_dxL_f;       // same for the L axis (set by k_finalize)
   int fp64_widths;               // 1: log-width positions always in FP64
   int min_rms;                   // 1: min-RMS weights
+  int noneq;                     // 1: two-temperature (Tvib, Trot) line strengths
+  double c_vib, c_rot, c_ref;    // -hc_k/Tvib, -hc_k/Trot, hc_k/Tref
   float mr_dv, mr_dxG, mr_dxL;   // wstep, dxG, dxL of the min-RMS corrections
   float mr_c1gg, mr_c2gg, mr_c1lg, mr_c2lg;
   float out_scale;               // 1/(L*wstep)

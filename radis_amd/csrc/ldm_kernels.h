@@ -438,6 +438,7 @@ struct LineRaw {
   double nu0;
   float dl, ga, gs, na, ns, S, El;
   int iso;
+  float evl, evu;  // vibrational energies of the lower/upper state (two-temperature requests)
 };
 
 // Column pointers with the optional columns made unconditional: a missing
@@ -449,13 +450,20 @@ struct LineCols {
   const double* nu0;
   const float *S, *El, *ga, *gs, *na, *ns, *dl;
   const uint8_t* iso;
+  const float *evl, *evu;
   long long dmask, imask;
   __device__ explicit LineCols(const LdmDb& db)
       : nu0(db.nu0), S(db.S_ref), El(db.El), ga(db.g_air), gs(db.g_self), na(db.n_air),
         ns(db.n_self ? db.n_self : db.n_air), dl(db.delta ? db.delta : db.dflt),
-        iso(db.iso ? db.iso : reinterpret_cast<const uint8_t*>(db.dflt + 1)),
+        iso(db.iso ? db.iso : reinterpret_cast<const uint8_t*>(db.dflt + 1)), evl(db.evib_l), evu(db.evib_u),
         dmask(db.delta ? -1LL : 0LL), imask(db.iso ? -1LL : 0LL) {}
+  // NONEQ: also the vibrational energies (the request checked they exist)
+  template <bool NONEQ = false>
   __device__ __forceinline__ void load(long long i, LineRaw& r) const {
+    if (NONEQ) {
+      r.evl = evl[i];
+      r.evu = evu[i];
+    }
     r.iso = iso[i & imask];  // first: the per-isotopologue lookups wait on it alone
     r.nu0 = nu0[i];
     r.dl = dl[i & dmask];
@@ -491,13 +499,15 @@ struct LineConst {
   float c_stim_T, c_stim_R, log_r_f, pa_air_f, pa_self_f;
   float laG0_f, inv_dxG_f, laL0_f, inv_dxL_f;
   int nv, fp64_widths, has_delta, min_rms;
+  double c_vib, c_rot, c_ref;
   const LdmJob* job;
   __device__ explicit LineConst(const LdmJob& J, const LdmDb& db)
       : wmin(J.wmin), delta_w(J.delta_w), inv_delta_w(J.inv_delta_w), p_atm(J.p_atm), x_air(J.x_air),
         x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
         c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
         laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
-        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), job(&J) {}
+        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), c_vib(J.c_vib),
+        c_rot(J.c_rot), c_ref(J.c_ref), job(&J) {}
 };
 
 // min-RMS weights, radis/lbl/broadening.py:2265-2303: the Gaussian and
@@ -555,6 +565,7 @@ __device__ __forceinline__ double fwhm_lorentz_c(const LineConst& C, double g_ai
   return (air + slf) * 2.0;
 }
 
+template <bool NONEQ = false>
 __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r, const IsoVals& iv,
                                            const LineTables& T, int ng, int nl, LineTerms& t) {
   const double nu0 = r.nu0;
@@ -602,11 +613,27 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   // for |x| >= 0.1 the difference 1 - e^x loses under 4 bits, so the fast
   // exponential and division (~2 ulp) keep ~1e-6 relative accuracy; smaller
   // arguments (far-infrared lines, hot gas) take expm1f
-  const float xT = C.c_stim_T * nuf, xR = C.c_stim_R * nuf;
-  float stim;
-  if (xT < -0.1f && xR < -0.1f) stim = __fdividef(1.0f - __expf(xT), 1.0f - __expf(xR));
-  else stim = expm1f(xT) / expm1f(xR);
-  t.I = boltzmann_scale(s64, C.c_boltz * (double)r.El) * stim;
+  if (NONEQ) {
+    // two-temperature populations, radis/lbl/base.py:3479-3530 and
+    // 3937-3947: S0/Q_neq * [e^{-c2(Evl/Tv + Erl/Tr)} - e^{-c2(Evu/Tv + Eru/Tr)}]
+    // with Er = E - Ev and E_u = E_l + nu0, written as the equilibrium form
+    // S_ref Q(Tref)/Q_neq e^{x} (1 - e^{y}) / (1 - e^{-c2 nu0/Tref}), where
+    // x = -c2(Evl/Tv + Erl/Tr - El/Tref) and
+    // y = -c2((Evu-Evl)/Tv + (nu0-(Evu-Evl))/Tr)  (y > 0: population inversion, S < 0)
+#pragma clang fp contract(off)
+    const double el = (double)r.El, evl = (double)r.evl, dev = (double)r.evu - evl;
+    const double x = (C.c_vib * evl + C.c_rot * (el - evl)) + C.c_ref * el;
+    const float y = (float)(C.c_vib * dev + C.c_rot * (nu0 - dev));
+    const float xR = C.c_stim_R * nuf;
+    const float stim = expm1f(y) / expm1f(xR);
+    t.I = boltzmann_scale(s64, x) * stim;
+  } else {
+    const float xT = C.c_stim_T * nuf, xR = C.c_stim_R * nuf;
+    float stim;
+    if (xT < -0.1f && xR < -0.1f) stim = __fdividef(1.0f - __expf(xT), 1.0f - __expf(xR));
+    else stim = expm1f(xT) / expm1f(xR);
+    t.I = boltzmann_scale(s64, C.c_boltz * (double)r.El) * stim;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -626,16 +653,10 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 // sits on the common path (only atomics), so atomics retire in the background.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
-                                                     const LdmAxes* __restrict__ axes,
-                                                     const int4* __restrict__ items, LdmDb db) {
-  extern __shared__ __align__(16) double acc[];
-  __shared__ LineTables tab;
-  const int4 it = items[blockIdx.x];
-  const LdmJob& J = jobs[it.x];
-  const LdmAxes& A = axes[it.x];
+template <bool NONEQ>
+__device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, const int4 it, const LdmDb& db,
+                                          double* acc, LineTables& tab) {
   const int M = A.m;
-  if (M == 0 || J.status) return;
   const int ng = A.ng, nl = A.nl;
   const int W = J.tile_w, Wp = W + LDM_FILL_PAD;
   const long long k_lo = (long long)it.y * W;
@@ -656,10 +677,10 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   IsoVals v1;
   const long long first = lo + threadIdx.x;
   if (first < hi) {
-    cols.load(first, n1);
+    cols.load<NONEQ>(first, n1);
     iso_vals(J, n1.iso, v1);
     const long long l1 = first + LDM_FILL_T;
-    cols.load(l1 < hlast ? l1 : hlast, n2);
+    cols.load<NONEQ>(l1 < hlast ? l1 : hlast, n2);
   }
   {
     const int n2 = (M * Wp) >> 1;  // Wp is even: whole 16-byte pairs
@@ -676,9 +697,9 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
     n1 = n2;
     iso_vals(J, n1.iso, v1);
     const long long i2 = i + 2 * LDM_FILL_T;
-    cols.load(i2 < hlast ? i2 : hlast, n2);
+    cols.load<NONEQ>(i2 < hlast ? i2 : hlast, n2);
     LineTerms t;
-    line_terms(C, cur, cv, tab, ng, nl, t);
+    line_terms<NONEQ>(C, cur, cv, tab, ng, nl, t);
     const int c0 = t.k0 - (int)k_lo;  // column of corner k0 in the tile, in [-1, W-1] if touching
     if (c0 < -1 || c0 > wlen - 1) continue;
     // the 8 corners are base + {0, 1} + {0, nl*Wp} + {0, Wp}
@@ -722,6 +743,19 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
         for (int q = 0; q < wlen - c; ++q) dst[q] = (float)src[q];
     }
   }
+}
+
+__global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
+                                                     const LdmAxes* __restrict__ axes,
+                                                     const int4* __restrict__ items, LdmDb db) {
+  extern __shared__ __align__(16) double acc[];
+  __shared__ LineTables tab;
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  const LdmAxes& A = axes[it.x];
+  if (A.m == 0 || J.status) return;
+  if (J.noneq) fill_tile<true>(J, A, it, db, acc, tab);
+  else fill_tile<false>(J, A, it, db, acc, tab);
 }
 
 // four-step twiddle W_L^e (forward sign)
@@ -978,11 +1012,17 @@ __global__ __launch_bounds__(256) void k_line_terms(const LdmJob* __restrict__ j
   const long long n = J.i1 - J.i0;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
     LineRaw r;
-    cols.load(J.i0 + q, r);
     IsoVals iv;
-    iso_vals(J, r.iso, iv);
     LineTerms t;
-    line_terms(C, r, iv, tab, A.ng, A.nl, t);
+    if (J.noneq) {
+      cols.load<true>(J.i0 + q, r);
+      iso_vals(J, r.iso, iv);
+      line_terms<true>(C, r, iv, tab, A.ng, A.nl, t);
+    } else {
+      cols.load(J.i0 + q, r);
+      iso_vals(J, r.iso, iv);
+      line_terms(C, r, iv, tab, A.ng, A.nl, t);
+    }
     k0[q] = t.k0; l0[q] = t.l0; m0[q] = t.m0;
     tv[q] = t.tv; tg[q] = t.tg; tl[q] = t.tl; I[q] = t.I;
   }

@@ -88,7 +88,9 @@ class LineDatabase:
 
     ``lines`` carries ``nu0, S_ref, El, g_air, g_self, n_air`` and optionally
     ``n_self, delta, iso``, plus ``molar_mass`` (g/mol per iso slot, slot 0
-    unused) and a partition function ``Q(iso, T)``.  Lines are sorted by
+    unused) and a partition function ``Q(iso, T)``.  Two-temperature requests
+    also need the vibrational energies ``Evib_l, Evib_u`` of the lower and
+    upper states and ``Qneq(iso, Tvib, Trot)``.  Lines are sorted by
     ``nu0`` here if needed; ``order`` maps device rows to input rows."""
 
     def __init__(self, device: Device, lines):
@@ -112,6 +114,9 @@ class LineDatabase:
             cols[k] = None if v is None else _F32(pick(np.asarray(v)))
         iso = get("iso")
         cols["iso"] = None if iso is None else np.ascontiguousarray(pick(np.asarray(iso)), dtype=np.uint8)
+        for k in ("Evib_l", "Evib_u"):
+            v = get(k)
+            cols[k] = None if v is None else _F32(pick(np.asarray(v)))
         mm = np.zeros(MAX_ISO)
         mma = np.asarray(get("molar_mass"), dtype=np.float64).ravel()
         if mma.size > MAX_ISO:
@@ -120,12 +125,14 @@ class LineDatabase:
         self.n_iso_slots = max(2, int(mma.size))
         self.molar_mass = mm
         self.Q = get("Q")
+        self.Qneq = get("Qneq")
         self.order = order
         self.nu0 = nu0
         self.n = int(nu0.size)
         self.has_iso = cols["iso"] is not None
         self.has_delta = cols["delta"] is not None
         self.has_nself = cols["n_self"] is not None
+        self.has_evib = cols["Evib_l"] is not None
         self._keep = (nu0, cols, mm)
         st = _native.Lines()
         fp = lambda a: None if a is None else a.ctypes.data_as(POINTER(c_float))
@@ -137,6 +144,7 @@ class LineDatabase:
         st.n = self.n
         st.molar_mass = mm.ctypes.data_as(POINTER(ctypes.c_double))
         st.n_iso_slots = self.n_iso_slots
+        st.Evib_l, st.Evib_u = fp(cols["Evib_l"]), fp(cols["Evib_u"])
         h = c_void_p()
         check(self._lib.radis_ldm_db_upload(device.handle, byref(st), byref(h)))
         self._h = h
@@ -169,6 +177,16 @@ class LineDatabase:
                 out[s] = float(self.Q(s, Tref)) / float(self.Q(s, T))
         return out
 
+    def q_ratio_noneq(self, Tvib, Trot, Tref):
+        """Q(Tref)/Q_neq(Tvib, Trot) per iso slot; slots without lines get 1."""
+        if self.Qneq is None or self.Q is None:
+            raise ValueError("two-temperature requests need Q(iso, T) and Qneq(iso, Tvib, Trot) with the lines")
+        out = np.ones(MAX_ISO)
+        for s in range(1, self.n_iso_slots):
+            if self.molar_mass[s] > 0:
+                out[s] = float(self.Q(s, Tref)) / float(self.Qneq(s, Tvib, Trot))
+        return out
+
     def close(self):
         if getattr(self, "_h", None) is not None:
             self._lib.radis_ldm_db_destroy(self._h)
@@ -184,7 +202,7 @@ class LineDatabase:
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
                  fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0,
-                 generic_fft=False, optimization="simple"):
+                 generic_fft=False, optimization="simple", Tvib=None, Trot=None):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
@@ -192,20 +210,33 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     fix the FFT split and the fill tile width (default: the autotuner's cache,
     else the planner); ``generic_fft`` selects the run-time-sized FFT passes;
     ``optimization`` is the LDM weighting, ``"simple"`` or ``"min-RMS"``
-    (radis/lbl/broadening.py:2265-2318)."""
+    (radis/lbl/broadening.py:2265-2318).  ``Tvib`` and ``Trot`` select
+    two-temperature line strengths (``Tgas`` is then the translational
+    temperature of the widths; RADIS's ``non_eq_spectrum`` takes
+    ``Ttrans = Trot`` by default, radis/lbl/factory.py:1685-1687)."""
     if optimization not in ("simple", "min-RMS"):
         raise ValueError(f"optimization must be 'simple' or 'min-RMS', not {optimization!r}")
     r = _native.Request()
     r.wmin, r.wmax, r.wstep = float(wmin), float(wmax), float(wstep)
     r.Tgas, r.pressure_bar, r.mole_fraction = float(Tgas), float(pressure_bar), float(mole_fraction)
     r.Tref, r.dxG, r.dxL, r.folding_thresh = float(Tref), float(dxG), float(dxL), float(folding_thresh)
-    q = db.q_ratio(Tgas, Tref) if q_ratio is None else np.asarray(q_ratio, dtype=np.float64)
+    noneq = Tvib is not None or Trot is not None
+    if noneq and (Tvib is None or Trot is None):
+        raise ValueError("give both Tvib and Trot")
+    if q_ratio is not None:
+        q = np.asarray(q_ratio, dtype=np.float64)
+    elif noneq:
+        q = db.q_ratio_noneq(Tvib, Trot, Tref)
+    else:
+        q = db.q_ratio(Tgas, Tref)
     for s in range(MAX_ISO):
         r.q_ratio[s] = float(q[s]) if s < q.size else 1.0
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
     r.flags = ((_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
-               | (_native.MIN_RMS if optimization == "min-RMS" else 0))
+               | (_native.MIN_RMS if optimization == "min-RMS" else 0) | (_native.NONEQ if noneq else 0))
+    if noneq:
+        r.Tvib, r.Trot = float(Tvib), float(Trot)
     r.fft_n1, r.tile_w = int(fft_n1 or 0), int(tile_w or 0)
     if line_window is not None:
         r.line_lo, r.line_hi = float(line_window[0]), float(line_window[1])

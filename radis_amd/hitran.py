@@ -3,7 +3,8 @@
 Fixed-width fields (HITRAN 2004 format, as read by RADIS's ``hit2df``,
 radis/api/hitranapi.py): molecule (2), isotopologue (1), nu (12), S (10),
 A (10), gamma_air (5), gamma_self (5), E'' (10), n_air (4), delta_air (8),
-and the upper-state weight g' (7) at columns 147-153.
+and, where present, the global quanta (15 + 15, columns 68-97) and the
+statistical weights g', g'' (7 + 7, columns 147-160).
 """
 from __future__ import annotations
 
@@ -27,14 +28,22 @@ _FIELDS = [
     ("n_air", 55, 59, float),
     ("delta", 59, 67, float),
 ]
-# upper-state statistical weight g' (HITRAN 2004 columns 147-153), read when present
-_GU = ("gu", 146, 153, float)
+# read when present: upper/lower global quanta (columns 68-82, 83-97) and
+# upper/lower statistical weights g', g'' (columns 147-153, 154-160)
+_GQ_U, _GQ_L = (67, 82), (82, 97)
+_GU, _GL = (146, 153), (153, 160)
 
 
 def read_par(path):
     """Columns of a .par file as a dict of numpy arrays (file order)."""
     cols = {k: [] for k, *_ in _FIELDS}
-    gu = []
+    extra = {"gu": [], "gl": [], "gq_u": [], "gq_l": []}
+
+    def field(rec, ab, conv, missing):
+        a, b = ab
+        txt = rec[a:b] if len(rec) >= b else ""
+        return conv(txt) if txt.strip() else missing
+
     with open(path) as f:
         for line in f:
             rec = line.rstrip("\n")
@@ -42,9 +51,15 @@ def read_par(path):
                 continue
             for k, a, b, conv in _FIELDS:
                 cols[k].append(conv(rec[a:b]))
-            gu.append(float(rec[_GU[1]:_GU[2]]) if len(rec) >= _GU[2] and rec[_GU[1]:_GU[2]].strip() else np.nan)
+            extra["gu"].append(field(rec, _GU, float, np.nan))
+            extra["gl"].append(field(rec, _GL, float, np.nan))
+            extra["gq_u"].append(field(rec, _GQ_U, str.strip, ""))
+            extra["gq_l"].append(field(rec, _GQ_L, str.strip, ""))
     out = {k: np.asarray(v) for k, v in cols.items()}
-    out["gu"] = np.asarray(gu, dtype=np.float64)
+    out["gu"] = np.asarray(extra["gu"], dtype=np.float64)
+    out["gl"] = np.asarray(extra["gl"], dtype=np.float64)
+    out["gq_u"] = np.asarray(extra["gq_u"])
+    out["gq_l"] = np.asarray(extra["gq_l"])
     out["iso"] = out["iso"].astype(np.int64)
     return out
 

@@ -311,3 +311,69 @@ def test_min_rms_weights_lower_the_error_against_voigt():
     assert med <= 0.5 and worst < 1.0, (med, worst)
     med, worst = _rms_ratio("doppler")
     assert med <= 0.14 and worst < 0.3, (med, worst)
+
+
+# --- two-temperature line strengths: lbl/base.py:3479-3530, 3937-3947 ----------
+def _co_levels(vmax=6, jmax=80, we=2169.81, wexe=13.29, B=1.9313):
+    """Toy CO level set: anharmonic vibration, rigid rotor, g = 2J+1."""
+    v, J = np.meshgrid(np.arange(vmax + 1), np.arange(jmax + 1), indexing="ij")
+    G = we * (v + 0.5) - wexe * (v + 0.5) ** 2
+    Ev = G - (we * 0.5 - wexe * 0.25)
+    Er = B * J * (J + 1)
+    return v.ravel(), J.ravel(), Ev.ravel(), Er.ravel(), (2 * J + 1).ravel().astype(float)
+
+
+def _qneq(Ev, Er, g, Tvib, Trot):
+    return float(np.sum(g * np.exp(-o.hc_k * (Ev / Tvib + Er / Trot))))
+
+
+def test_noneq_populations_sum_to_one():
+    """Populations of every level of a closed level set sum to 1 when
+    Q_neq is the sum over those levels (brute force)."""
+    v, J, Ev, Er, g = _co_levels()
+    for Tvib, Trot in [(3000.0, 300.0), (800.0, 2500.0)]:
+        Q = _qneq(Ev, Er, g, Tvib, Trot)
+        nu, nl = o.populations_noneq(1.0, g, Ev, Er, 1.0, g, Ev, Er, Tvib, Trot, Q)
+        assert np.isclose(nl.sum(), 1.0, rtol=1e-12) and np.allclose(nu, nl)
+
+
+def test_noneq_at_one_temperature_is_the_equilibrium_strength():
+    """Tvib = Trot = T and Q_neq = Q(T): S_noneq equals the equilibrium
+    linestrength_eq from S(Tref) (which carries the Einstein-A S0 through
+    Q(Tref), the Boltzmann factor and the stimulated-emission term)."""
+    v, J, Ev, Er, g = _co_levels(vmax=3, jmax=40)
+    rng = np.random.default_rng(5)
+    lo = rng.choice(np.flatnonzero(v < 3), 300)
+    up = lo + (J.max() + 1) + rng.choice([-1, 1], 300)  # v+1, J+-1
+    up = np.clip(up, 0, v.size - 1)
+    ok = (v[up] == v[lo] + 1)
+    lo, up = lo[ok], up[ok]
+    El, Eu = Ev[lo] + Er[lo], Ev[up] + Er[up]
+    nu0 = Eu - El
+    A = rng.uniform(1.0, 50.0, lo.size)
+    S0 = o.s0_einstein(A, g[up], nu0, 0.9865)
+    Qf = lambda T: _qneq(Ev, Er, g, T, T)  # noqa: E731
+    lines = dict(S0=S0, El=El, gl=g[lo], gu=g[up], Evib_l=Ev[lo], Evib_u=Ev[up], Qneq=lambda i, a, b: _qneq(Ev, Er, g, a, b))
+    for T in (300.0, 1200.0, 3500.0):
+        S_neq = o._noneq_strength(lines, nu0, None, T, T, 296.0)
+        S_ref = S0 * np.exp(-o.hc_k * El / 296.0) * (1 - np.exp(-o.hc_k * nu0 / 296.0)) / Qf(296.0)
+        S_eq = o.linestrength_eq(S_ref, El, nu0, T, 296.0, Qf(296.0) / Qf(T))
+        np.testing.assert_allclose(S_neq, S_eq, rtol=1e-12)
+        # the S_ref route of the restatement gives the same
+        lines_ref = dict(lines, S0=None, S_ref=S_ref, Q=lambda i, TT: Qf(TT))
+        np.testing.assert_allclose(o._noneq_strength(lines_ref, nu0, None, T, T, 296.0), S_eq, rtol=1e-12)
+
+
+def test_noneq_inversion_gives_negative_strength():
+    """S has the sign of nl/gl - nu/gu: with Tvib >> Trot a P-branch line
+    (upper rotational energy below the lower) is inverted and absorbs
+    negatively; its R-branch partner does not."""
+    v, J, Ev, Er, g = _co_levels()
+    Tvib, Trot = 8000.0, 80.0
+    Q = _qneq(Ev, Er, g, Tvib, Trot)
+    i_l = np.flatnonzero((v == 0) & (J == 30))[0]
+    for dJ, sign in ((-1, -1.0), (+1, 1.0)):
+        i_u = np.flatnonzero((v == 1) & (J == 30 + dJ))[0]
+        nu, nl = o.populations_noneq(1.0, g[i_l], Ev[i_l], Er[i_l], 1.0, g[i_u], Ev[i_u], Er[i_u], Tvib, Trot, Q)
+        S = o.linestrength_noneq(1.0, g[i_l], g[i_u], nu, nl)
+        assert np.sign(S) == sign and np.sign(nl / g[i_l] - nu / g[i_u]) == sign
