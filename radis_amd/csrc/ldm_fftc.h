@@ -372,10 +372,18 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ 
   fc_load_tw<N1, FC_T>(s_tw, J.tw1);
   const float sc = J.out_scale;
   auto ld = [&](int, int, int e) -> float2 { return J.bbuf[(long long)(e >> LB) * N2 + n2_0 + (e & (B - 1))]; };
-  auto st = [&](int, int, int e, float2 z) {
-    const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-    if (n < nv) store_output(J, n, z.x * sc);
-  };
   auto nop = [] {};
-  fc_fft_io<N1, B, FC_T, false, true, true>(buf, s_tw, ld, st, nop);
+  if (J.derived) {  // uniform per workgroup: the plain store path stays as lean as without the epilogue
+    auto st = [&](int, int, int e, float2 z) {
+      const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
+      if (n < nv) store_derived(J, n, z.x * sc);
+    };
+    fc_fft_io<N1, B, FC_T, false, true, true>(buf, s_tw, ld, st, nop);
+  } else {
+    auto st = [&](int, int, int e, float2 z) {
+      const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
+      if (n < nv) J.out[n] = z.x * sc;
+    };
+    fc_fft_io<N1, B, FC_T, false, true, true>(buf, s_tw, ld, st, nop);
+  }
 }

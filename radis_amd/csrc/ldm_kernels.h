@@ -935,16 +935,15 @@ __global__ __launch_bounds__(T, 768 / T) void k_fft_b(const LdmJob* __restrict__
   }
 }
 
-// abscoeff (times 2^-scale_log2) at grid point n and, when requested, the
-// derived quantities (FP64): radis/lbl/factory.py:1289-1297,
+// abscoeff (times 2^-scale_log2) at grid point n and the requested derived
+// quantities (FP64): radis/lbl/factory.py:1289-1297,
 // radis/spectrum/equations.py:21-44, radis/phys/blackbody.py:35-74.
 #define LDM_H_SI 6.62607015e-34      /* radis/phys/constants.py:22,33,39 */
 #define LDM_C_SI 2.99792458e8
 #define LDM_KB_SI 1.3806490e-23
-__device__ __forceinline__ void store_output(const LdmJob& J, int n, float v) {
+__device__ __forceinline__ void store_derived(const LdmJob& J, int n, float v) {
 #pragma clang fp contract(off)
   J.out[n] = v;
-  if (!J.derived) return;
   const double a_scaled = (double)v * J.path_length;   // absorbance times 2^-scale_log2
   const double A = ldexp(a_scaled, J.scale_log2);
   const double emis = -expm1(-A);
@@ -986,10 +985,14 @@ __global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_c(const LdmJob* __restrict
   __syncthreads();
   lds_fft<false, LDM_CAP, LDM_FFT_T>(buf, N1, bs, J.rad1, J.nrad1, J.tw1);
   const float sc = J.out_scale;
+  const bool derived = J.derived != 0;
   for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
     const int n1 = e >> bs, b = e & (B - 1);
     const long long n = n2_0 + b + (long long)N2 * n1;
-    if (n < J.nv) store_output(J, (int)n, buf[pad_idx(e)].x * sc);
+    if (n < J.nv) {
+      if (derived) store_derived(J, (int)n, buf[pad_idx(e)].x * sc);
+      else J.out[n] = buf[pad_idx(e)].x * sc;
+    }
   }
 }
 
