@@ -50,7 +50,9 @@ def parse():
 def stage_bytes(n_lines, info, db):
     """Algorithmic HBM bytes per launch of each kernel of one request.
 
-    axis        : 12 B/line (g_air, g_self, n_air f32) [+4 n_self] [+9 nu0 f64 + iso u8 with several isotopologues]
+    axis        : 12 B per scanned line (g_air, g_self, n_air f32) [+4 n_self] [+9 nu0 f64 + iso u8 with several
+                  isotopologues]; the lines of whole blocks are replaced by their width candidates, 16 B each
+                  (+ 2 nu0 per block and isotopologue with several)
     tile_bounds : 0 (a few dozen probes of nu0 per tile: latency, not bytes)
     fill        : 28 B/line (nu0 f64; S_ref, El, g_air, g_self, n_air f32) [+4 n_self] [+4 delta] [+1 iso]
                   + 4*M*(N_v+1) LDM write
@@ -65,7 +67,7 @@ def stage_bytes(n_lines, info, db):
     ns = 4 if db.has_nself else 0
     per_fill = 28 + ns + (4 if db.has_delta else 0) + (1 if db.has_iso else 0)
     return {
-        "axis": n_lines * (12 + ns + (9 if db.has_iso else 0)),
+        "axis": int(info.axis_scanned) * (12 + ns + (9 if db.has_iso else 0)) + 16 * int(info.axis_candidates),
         "finalize": 0,
         "tile_bounds": 0,
         "fill": n_lines * per_fill + 4 * M * V,

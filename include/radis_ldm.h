@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RADIS_LDM_ABI_VERSION 2
+#define RADIS_LDM_ABI_VERSION 3
 
 enum {
     RADIS_LDM_OK = 0,
@@ -113,6 +113,9 @@ typedef struct {
  * distributions, radis/lbl/base.py:3479-3530 populations and 3937-3947
  * calc_linestrength_noneq); the line list must carry Evib_l and Evib_u. */
 #define RADIS_LDM_NONEQ 8
+/* Validation: find the width extremes by scanning every window line instead
+ * of the per-block candidates (same result, bit for bit). */
+#define RADIS_LDM_SCAN_AXIS 16
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {
@@ -126,6 +129,11 @@ typedef struct {
     int32_t n_g, n_l; /* device-computed LDM axis sizes (valid after a run has completed) */
     int32_t status;   /* device status of the last completed run (0 = ok) */
     int32_t compiled_fft; /* 1: the FFT passes are the compile-time-sized kernels for (n1, n2) */
+    double wg_min, wg_max, wl_min, wl_max; /* Gaussian / Lorentzian FWHM extremes over the window
+                                 that the LDM axes of the last completed run start from
+                                 (radis/lbl/broadening.py:1823-1837)                       */
+    int64_t axis_scanned;     /* window lines the width reduction reads in full          */
+    int64_t axis_candidates;  /* per-block width candidates it reads (16 B each) instead  */
 } radis_ldm_request_info;
 
 /* --- context -------------------------------------------------------------- */

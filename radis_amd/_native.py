@@ -87,7 +87,8 @@ FP64_WIDTHS = 1
 GENERIC_FFT = 2
 MIN_RMS = 4
 NONEQ = 8
-ABI_VERSION = 2
+SCAN_AXIS = 16
+ABI_VERSION = 3
 
 
 class RequestInfo(ctypes.Structure):
@@ -105,6 +106,12 @@ class RequestInfo(ctypes.Structure):
         ("n_l", c_int32),
         ("status", c_int32),
         ("compiled_fft", c_int32),
+        ("wg_min", c_double),
+        ("wg_max", c_double),
+        ("wl_min", c_double),
+        ("wl_max", c_double),
+        ("axis_scanned", c_int64),
+        ("axis_candidates", c_int64),
     ]
 
 

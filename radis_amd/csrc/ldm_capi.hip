@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "radis_ldm.h"
@@ -106,6 +107,8 @@ struct radis_ldm_db {
   // per-block statistics used for workspace and scale bounds
   std::vector<float> ga_min, ga_max, gs_min, gs_max, na_min, na_max, ns_min, ns_max, e_min, e_max, s_max;
   std::vector<float> dv_min, dv_max;  // Evib_u - Evib_l (two-temperature bound)
+  bool has_sky = false;                // Lorentzian-width candidates per block (k_axis)
+  std::vector<int> h_sky_off;          // host copy of the candidate offsets
 };
 
 namespace {
@@ -196,6 +199,7 @@ struct Req {
   int64_t stride = 0;
   int scale_log2 = 0;
   int bA = 0, bB = 0;
+  int n_axp = 0;       // axis partials (items of k_axis)
   size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
 };
@@ -547,6 +551,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.row_b = (float*)(work + R.off_rows + (size_t)8 * R.m_cap);
   J.row_inv_f0 = (float*)(work + R.off_rows + (size_t)12 * R.m_cap);
   J.ax_part = (unsigned long long*)(work + R.off_axp);
+  J.n_ax_part = R.n_axp;
   J.tw1 = R.tw1;
   J.tw2 = R.tw2;
   J.twl_hi = R.twl_hi;
@@ -596,7 +601,8 @@ int build_layout(radis_ldm_batch* b) {
     R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
     R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
     R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
-    R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK) * 32, 256);
+    // axis partials: at most one per chunk, plus the two partial blocks at the window edges
+    R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2) * 32, 256);
   }
   // --- twiddle tables, deduplicated by size
   std::map<int, size_t> sub_off;
@@ -661,7 +667,29 @@ int build_layout(radis_ldm_batch* b) {
   for (int j = 0; j < b->n; ++j) {
     const Req& R = b->reqs[j];
     const int64_t nl = R.i1 - R.i0;
-    for (int64_t c = 0; c * LDM_AX_CHUNK < nl; ++c) ax.push_back(make_int4(j, (int)c, 0, 0));
+    {
+      // axis items: lines of partial blocks scanned, whole blocks through
+      // their width candidates (LDM_AX_CHUNK == LDM_SKY_BLOCK)
+      int np = 0;
+      auto scan = [&](int64_t a, int64_t e) {
+        for (int64_t c = a; c < e; c += LDM_AX_CHUNK)
+          ax.push_back(make_int4(j, np++, (int)(c - R.i0), (int)std::min<int64_t>(LDM_AX_CHUNK, e - c)));
+      };
+      const int64_t fb0 = (R.i0 + LDM_SKY_BLOCK - 1) / LDM_SKY_BLOCK, fb1 = R.i1 / LDM_SKY_BLOCK;
+      if (b->db->has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
+        scan(R.i0, fb0 * LDM_SKY_BLOCK);
+        // at least LDM_SKY_GROUP_MIN blocks per workgroup (each partial costs
+        // the reduction a read), and at most about LDM_SKY_WGS workgroups
+        const int64_t G = std::max<int64_t>(LDM_SKY_GROUP_MIN, (fb1 - fb0 + LDM_SKY_WGS - 1) / LDM_SKY_WGS);
+        for (int64_t g = fb0; g < fb1; g += G)
+          ax.push_back(make_int4(j, np++, (int)g, -(int)std::min<int64_t>(G, fb1 - g)));
+        scan(fb1 * LDM_SKY_BLOCK, R.i1);
+      } else {
+        scan(R.i0, R.i1);
+      }
+      b->reqs[j].n_axp = np;
+    }
+    (void)nl;
     for (int t = 0; t < R.n_tiles; ++t) fi.push_back(make_int4(j, t, 0, 0));
     const int pcap = (R.m_cap + 1) / 2;
     const int nA = R.plan.n2 >> R.bA;
@@ -929,6 +957,44 @@ int radis_ldm_plan_grid(double wmin, double wmax, double wstep, int64_t* n_v, in
   return 0;
 }
 
+namespace {
+
+// Pareto-maximal lines of one block for monotonicity direction d (see
+// k_axis): d0 maximises (g_air, g_self, -n_air, -n_self), d1 (-g, +n),
+// d2 (+g, +n), d3 (-g, -n).  Sort-filter-skyline: visit the lines by
+// decreasing coordinate sum (a dominating line never comes later, up to
+// rounding of the sum, which can only keep a dominated line too) and keep
+// those no kept line dominates.
+void block_skyline(const radis_ldm_lines* L, int64_t i0, int cnt, int d, std::vector<float4>& out) {
+  const float sg = (d == 0 || d == 2) ? 1.0f : -1.0f;
+  const float sn = (d == 1 || d == 2) ? 1.0f : -1.0f;
+  struct P { float c[4]; double key; int64_t i; };
+  std::vector<P> v((size_t)cnt);
+  for (int k = 0; k < cnt; ++k) {
+    const int64_t i = i0 + k;
+    const float na = L->n_air[i], ns = L->n_self ? L->n_self[i] : na;
+    P& p = v[(size_t)k];
+    p.c[0] = sg * L->g_air[i]; p.c[1] = sg * L->g_self[i]; p.c[2] = sn * na; p.c[3] = sn * ns;
+    p.key = (double)p.c[0] + (double)p.c[1] + (double)p.c[2] + (double)p.c[3];
+    p.i = i;
+  }
+  std::sort(v.begin(), v.end(), [](const P& a, const P& b) { return a.key > b.key; });
+  std::vector<const P*> sky;
+  for (const P& p : v) {
+    bool dom = false;
+    for (const P* q : sky)
+      if (q->c[0] >= p.c[0] && q->c[1] >= p.c[1] && q->c[2] >= p.c[2] && q->c[3] >= p.c[3]) { dom = true; break; }
+    if (!dom) sky.push_back(&p);
+  }
+  for (const P* q : sky) {
+    const int64_t i = q->i;
+    const float na = L->n_air[i];
+    out.push_back(make_float4(L->g_air[i], L->g_self[i], na, L->n_self ? L->n_self[i] : na));
+  }
+}
+
+}  // namespace
+
 int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_db** out) {
   if (!ctx || !L || !out) return fail(RADIS_LDM_EINVAL, "null argument");
   if (L->n <= 0) return fail(RADIS_LDM_EINVAL, "empty line list");
@@ -1000,13 +1066,57 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
       db->dv_max[b] = std::max(db->dv_max[b], d);
     }
   }
+  // Lorentzian-width candidates of the whole blocks (k_axis).  Only for
+  // finite, non-negative widths and exponents: otherwise the monotonicity the
+  // candidates rest on fails, and every line is scanned.
+  bool sky_ok = true;
+  for (int64_t i = 0; i < n && sky_ok; ++i) {
+    const float na = L->n_air[i], ns = L->n_self ? L->n_self[i] : na;
+    sky_ok = std::isfinite(na) && std::isfinite(ns) && L->g_air[i] >= 0 && L->g_self[i] >= 0 &&
+             std::isfinite(L->g_air[i]) && std::isfinite(L->g_self[i]) && L->nu0[i] >= 0;
+  }
+  std::vector<float4> sky_vals;
+  std::vector<int> sky_off;
+  std::vector<int2> iso_rng;
+  const int nfull = (int)(n / LDM_SKY_BLOCK);
+  if (sky_ok && nfull > 0) {
+    std::vector<std::vector<float4>> per((size_t)nfull * 4);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; ++t)
+      th.emplace_back([&, t] {
+        for (int b = (int)t; b < nfull; b += (int)nth)
+          for (int d = 0; d < 4; ++d) block_skyline(L, (int64_t)b * LDM_SKY_BLOCK, LDM_SKY_BLOCK, d, per[(size_t)b * 4 + d]);
+      });
+    for (auto& x : th) x.join();
+    sky_off.assign((size_t)4 * (nfull + 1), 0);
+    for (int d = 0; d < 4; ++d) {
+      sky_off[(size_t)d * (nfull + 1)] = (int)sky_vals.size();
+      for (int b = 0; b < nfull; ++b) {
+        const auto& v = per[(size_t)b * 4 + d];
+        sky_vals.insert(sky_vals.end(), v.begin(), v.end());
+        sky_off[(size_t)d * (nfull + 1) + b + 1] = (int)sky_vals.size();
+      }
+    }
+    if (db->has_iso) {
+      iso_rng.assign((size_t)nfull * L->n_iso_slots, make_int2(-1, -1));
+      for (int64_t i = 0; i < (int64_t)nfull * LDM_SKY_BLOCK; ++i) {
+        int2& r = iso_rng[(size_t)(i / LDM_SKY_BLOCK) * L->n_iso_slots + L->iso[i]];
+        if (r.x < 0) r.x = (int)i;
+        r.y = (int)i;
+      }
+    }
+    db->has_sky = true;
+    db->h_sky_off = sky_off;
+  }
   // device arrays: one allocation
-  size_t off[12], tot = 0;
-  const size_t sz[12] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
+  size_t off[15], tot = 0;
+  const size_t sz[15] = {(size_t)n * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4,
                          db->has_nself ? (size_t)n * 4 : 0, db->has_delta ? (size_t)n * 4 : 0,
                          db->has_iso ? (size_t)n : 0, 8,
-                         db->has_evib ? (size_t)n * 4 : 0, db->has_evib ? (size_t)n * 4 : 0};
-  for (int k = 0; k < 12; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
+                         db->has_evib ? (size_t)n * 4 : 0, db->has_evib ? (size_t)n * 4 : 0,
+                         sky_vals.size() * sizeof(float4), sky_off.size() * sizeof(int), iso_rng.size() * sizeof(int2)};
+  for (int k = 0; k < 15; ++k) { off[k] = tot; tot = align_up(tot + sz[k], 256); }
   if (ctx->budget > 0 && ctx->in_use + (int64_t)tot > ctx->budget) {
     delete db;
     return fail(RADIS_LDM_ENOMEM, "line list of %zu bytes exceeds the device budget", tot);
@@ -1022,9 +1132,9 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   char* m = (char*)db->mem;
   // defaults block: a float 0 (missing delta column) and a byte 1 (missing iso column)
   unsigned char dflt[8] = {0, 0, 0, 0, 1, 0, 0, 0};
-  const void* src[12] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso, dflt,
-                         L->Evib_l, L->Evib_u};
-  for (int k = 0; k < 12; ++k)
+  const void* src[15] = {L->nu0, L->S_ref, L->El, L->g_air, L->g_self, L->n_air, L->n_self, L->delta, L->iso, dflt,
+                         L->Evib_l, L->Evib_u, sky_vals.data(), sky_off.data(), iso_rng.data()};
+  for (int k = 0; k < 15; ++k)
     if (sz[k]) {
       hipError_t e = hipMemcpy(m + off[k], src[k], sz[k], hipMemcpyHostToDevice);
       if (e != hipSuccess) {
@@ -1044,6 +1154,11 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
   db->dev.dflt = (const float*)(m + off[9]);
   db->dev.evib_l = db->has_evib ? (const float*)(m + off[10]) : nullptr;
   db->dev.evib_u = db->has_evib ? (const float*)(m + off[11]) : nullptr;
+  db->dev.sky = db->has_sky ? (const float4*)(m + off[12]) : nullptr;
+  db->dev.sky_off = db->has_sky ? (const int*)(m + off[13]) : nullptr;
+  db->dev.sky_nb = db->has_sky ? nfull : 0;
+  db->dev.n_iso_slots = L->n_iso_slots;
+  db->dev.iso_rng = (db->has_sky && db->has_iso) ? (const int2*)(m + off[14]) : nullptr;
   *out = db;
   return 0;
 }
@@ -1261,6 +1376,26 @@ int radis_ldm_batch_info(radis_ldm_batch* b, int i, radis_ldm_request_info* info
   info->n_g = res.ng;
   info->n_l = res.nl;
   info->status = res.status;
+  info->wg_min = res.wg_min;
+  info->wg_max = res.wg_max;
+  info->wl_min = res.wl_min;
+  info->wl_max = res.wl_max;
+  // what k_axis reads for this request: scanned lines and width candidates
+  {
+    const radis_ldm_db& db = *b->db;
+    const int64_t fb0 = (R.i0 + LDM_SKY_BLOCK - 1) / LDM_SKY_BLOCK, fb1 = R.i1 / LDM_SKY_BLOCK;
+    if (db.has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
+      const int nb1 = db.dev.sky_nb + 1;
+      const int dmax = std::log(R.r.Tref / R.r.Tgas) > 0.0 ? 2 : 0;
+      int64_t cand = 0;
+      for (int d = dmax; d <= dmax + 1; ++d) cand += db.h_sky_off[(size_t)d * nb1 + fb1] - db.h_sky_off[(size_t)d * nb1 + fb0];
+      info->axis_candidates = cand;
+      info->axis_scanned = (R.i1 - R.i0) - (fb1 - fb0) * LDM_SKY_BLOCK;
+    } else {
+      info->axis_candidates = 0;
+      info->axis_scanned = R.i1 - R.i0;
+    }
+  }
   return 0;
 }
 

@@ -20,6 +20,10 @@
 #define LDM_FILL_T 256     // threads of a fill workgroup
 #define LDM_FILL_PAD 2     // LDS row stride of a fill tile: tile_w + LDM_FILL_PAD doubles (16-byte aligned rows)
 #define LDM_AX_CHUNK 4096  // lines per axis-reduction workgroup (16 per lane)
+#define LDM_SKY_BLOCK 4096  // lines per width-candidate block
+#define LDM_SKY_GROUP_MIN 8  // fewest candidate blocks per k_axis workgroup
+#define LDM_SKY_WGS 600       // most candidate workgroups of k_axis per request (about)
+static_assert(LDM_AX_CHUNK == LDM_SKY_BLOCK, "a partial block is one scan chunk");
 #define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
 #define LDM_NFOLD_MAX 100000
 
@@ -43,6 +47,16 @@ struct LdmDb {
   const float* dflt;     // device block {0.0f, then byte 1}: what a missing delta / iso column reads
   const float* evib_l;   // lower/upper-state vibrational energies (cm-1), may be null
   const float* evib_u;
+  // Lorentzian-width candidates of every LDM_SKY_BLOCK-line block: for each of
+  // the 4 monotonicity directions, the block's Pareto-maximal lines as
+  // (g_air, g_self, n_air, n_self); list d of block b is
+  // sky[sky_off[d*(sky_nb+1) + b] .. sky_off[d*(sky_nb+1) + b + 1]).  Null if
+  // the database has none (then every line is scanned).
+  const float4* sky;
+  const int* sky_off;
+  int sky_nb;
+  int n_iso_slots;
+  const int2* iso_rng;   // [sky_nb][n_iso_slots]: first/last line of each iso in the block (-1: none)
 };
 
 struct LdmJob {
@@ -97,7 +111,8 @@ struct LdmJob {
   const float2* tw2;             // exp(-2 pi i e/n2), e < n2
   const float2* twl_hi;          // W_L^(h*1024)
   const float2* twl_lo;          // W_L^(e), e < 1024
-  unsigned long long* ax_part;   // [chunks][4]: per-chunk width extremes (wl min, wl max, wg min, wg max) as bits
+  unsigned long long* ax_part;   // [n_ax_part][4]: per-item width extremes (wl min, wl max, wg min, wg max) as bits
+  int n_ax_part;
   // reduction targets and status, initialised by the host every run
   unsigned long long wl_min_bits, wl_max_bits, wg_min_bits, wg_max_bits;
   int status;
@@ -114,4 +129,5 @@ struct LdmAxes {
 // Per-request results copied back to the host after every run.
 struct LdmResult {
   int ng, nl, status, pad_;
+  double wg_min, wg_max, wl_min, wl_max;  // FWHM extremes the axes were built from
 };

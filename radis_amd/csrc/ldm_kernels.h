@@ -110,66 +110,112 @@ __device__ __forceinline__ unsigned long long double_to_bits(double d) {
 // k_axis: min/max of the Lorentzian (and, for several isotopologues, the
 // Gaussian) FWHM over each request's window; radis/lbl/broadening.py:1823-1852.
 // Positive doubles order like their bit patterns, so the extremes are reduced
-// as unsigned 64-bit integers: one partial per chunk, combined by k_finalize.
-// Item = (job, chunk of LDM_AX_CHUNK lines).
+// as unsigned 64-bit integers: one partial per item, combined by k_finalize.
+// Item = (job, partial index, z, w): w > 0 scans the w lines from J.i0 + z;
+// w < 0 takes the width candidates of the -w whole blocks from block z.
+//
+// The Lorentzian FWHM 2 p [(Tref/T)^n_air g_air (1-x) + (Tref/T)^n_self g_self x]
+// is non-decreasing in g_air and g_self and monotone in n_air and n_self, in
+// the direction of log(Tref/T).  Its maximum (minimum) over a block is
+// therefore reached on the block's Pareto-maximal lines for (g up, n in that
+// direction) ((g down, n against it)), and FP64 evaluation keeps that order
+// (rounding is monotone, and different float inputs move exp's argument by
+// far more than an ulp), so the extremes over the candidates are bit-identical
+// to those over every line.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                               LdmDb db) {
   const int4 it = items[blockIdx.x];
   LdmJob& J = jobs[it.x];
-  const long long a = J.i0 + (long long)it.y * LDM_AX_CHUNK;
-  long long e = a + LDM_AX_CHUNK;
-  if (e > J.i1) e = J.i1;
   unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
-  // FP32 widths of this lane's lines stay in registers; only the FP32 argmin
-  // and argmax are re-evaluated in FP64.  If another of the lane's lines is
-  // within the FP32 error bound of an extreme (a near tie), every line of the
-  // lane is evaluated in FP64 (rare), so the exact FP64 extremes are found.
-  constexpr int NPER = LDM_AX_CHUNK / 256;
-  float wf[NPER];
-  float fmin = 3.0e38f, fmax = -1.0f;
-  int kmin = -1, kmax = -1;
-#pragma unroll
-  for (int k = 0; k < NPER; ++k) {
-    const long long i = a + threadIdx.x + (long long)k * 256;
-    wf[k] = -1.0f;
-    if (i < e) {
-      const float na = db.n_air[i];
-      wf[k] = fwhm_lorentz_f32(J, db.g_air[i], db.g_self[i], na, db.n_self ? db.n_self[i] : na);
-      if (wf[k] < fmin) { fmin = wf[k]; kmin = k; }
-      if (wf[k] > fmax) { fmax = wf[k]; kmax = k; }
+  if (it.w < 0) {
+    const int gb0 = it.z, G = -it.w;
+    const int dmax = J.log_r > 0.0 ? 2 : 0, dmin = dmax + 1;
+    const int stride = db.sky_nb + 1;
+    {
+      const int* off = db.sky_off + dmax * stride;
+      for (int c = off[gb0] + (int)threadIdx.x; c < off[gb0 + G]; c += blockDim.x) {
+        const float4 q = db.sky[c];
+        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, q.x, q.y, q.z, q.w));
+        lmax = bl > lmax ? bl : lmax;
+      }
     }
-  }
-  bool tie = false;
-#pragma unroll
-  for (int k = 0; k < NPER; ++k) {
-    if (wf[k] >= 0.0f && k != kmin && wf[k] <= fmin * (1.0f + 2e-5f)) tie = true;
-    if (wf[k] >= 0.0f && k != kmax && wf[k] >= fmax * (1.0f - 2e-5f)) tie = true;
-  }
-  auto exact = [&](int k) {
-    const long long i = a + threadIdx.x + (long long)k * 256;
-    const double na = db.n_air[i];
-    return double_to_bits(fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, db.n_self ? (double)db.n_self[i] : na));
-  };
-  if (kmin >= 0) {
-    if (!tie) {
-      lmin = exact(kmin);
-      lmax = exact(kmax);
-    } else {
-      for (int k = 0; k < NPER; ++k)
-        if (wf[k] >= 0.0f) {
-          const unsigned long long bl = exact(k);
-          lmin = bl < lmin ? bl : lmin;
-          lmax = bl > lmax ? bl : lmax;
+    {
+      const int* off = db.sky_off + dmin * stride;
+      for (int c = off[gb0] + (int)threadIdx.x; c < off[gb0 + G]; c += blockDim.x) {
+        const float4 q = db.sky[c];
+        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, q.x, q.y, q.z, q.w));
+        lmin = bl < lmin ? bl : lmin;
+      }
+    }
+    if (!J.single_iso) {
+      // Gaussian FWHM = nu0 * const(iso): per isotopologue the block's first
+      // and last line (nu0 sorted)
+      const int ns = db.n_iso_slots;
+      for (int q = threadIdx.x; q < G * ns; q += blockDim.x) {
+        const int b = gb0 + q / ns, s = q - (q / ns) * ns;
+        const int2 r = db.iso_rng[(long long)b * ns + s];
+        if (r.x >= 0) {
+          const unsigned long long b0 = double_to_bits(fwhm_gauss(db.nu0[r.x], J.dopp[s]));
+          const unsigned long long b1 = double_to_bits(fwhm_gauss(db.nu0[r.y], J.dopp[s]));
+          gmin = b0 < gmin ? b0 : gmin;
+          gmax = b1 > gmax ? b1 : gmax;
         }
+      }
     }
-  }
-  if (!J.single_iso) {
-    for (long long i = a + threadIdx.x; i < e; i += blockDim.x) {
-      const int iso = db.iso ? db.iso[i] : 1;
-      const unsigned long long bg = double_to_bits(fwhm_gauss(db.nu0[i], J.dopp[iso]));
-      gmin = bg < gmin ? bg : gmin;
-      gmax = bg > gmax ? bg : gmax;
+  } else {
+    const long long a = J.i0 + it.z;
+    const long long e = a + it.w;
+    // FP32 widths of this lane's lines stay in registers; only the FP32 argmin
+    // and argmax are re-evaluated in FP64.  If another of the lane's lines is
+    // within the FP32 error bound of an extreme (a near tie), every line of the
+    // lane is evaluated in FP64 (rare), so the exact FP64 extremes are found.
+    constexpr int NPER = LDM_AX_CHUNK / 256;
+    float wf[NPER];
+    float fmin = 3.0e38f, fmax = -1.0f;
+    int kmin = -1, kmax = -1;
+#pragma unroll
+    for (int k = 0; k < NPER; ++k) {
+      const long long i = a + threadIdx.x + (long long)k * 256;
+      wf[k] = -1.0f;
+      if (i < e) {
+        const float na = db.n_air[i];
+        wf[k] = fwhm_lorentz_f32(J, db.g_air[i], db.g_self[i], na, db.n_self ? db.n_self[i] : na);
+        if (wf[k] < fmin) { fmin = wf[k]; kmin = k; }
+        if (wf[k] > fmax) { fmax = wf[k]; kmax = k; }
+      }
+    }
+    bool tie = false;
+#pragma unroll
+    for (int k = 0; k < NPER; ++k) {
+      if (wf[k] >= 0.0f && k != kmin && wf[k] <= fmin * (1.0f + 2e-5f)) tie = true;
+      if (wf[k] >= 0.0f && k != kmax && wf[k] >= fmax * (1.0f - 2e-5f)) tie = true;
+    }
+    auto exact = [&](int k) {
+      const long long i = a + threadIdx.x + (long long)k * 256;
+      const double na = db.n_air[i];
+      return double_to_bits(fwhm_lorentz(J, db.g_air[i], db.g_self[i], na, db.n_self ? (double)db.n_self[i] : na));
+    };
+    if (kmin >= 0) {
+      if (!tie) {
+        lmin = exact(kmin);
+        lmax = exact(kmax);
+      } else {
+        for (int k = 0; k < NPER; ++k)
+          if (wf[k] >= 0.0f) {
+            const unsigned long long bl = exact(k);
+            lmin = bl < lmin ? bl : lmin;
+            lmax = bl > lmax ? bl : lmax;
+          }
+      }
+    }
+    if (!J.single_iso) {
+      for (long long i = a + threadIdx.x; i < e; i += blockDim.x) {
+        const int iso = db.iso ? db.iso[i] : 1;
+        const unsigned long long bg = double_to_bits(fwhm_gauss(db.nu0[i], J.dopp[iso]));
+        gmin = bg < gmin ? bg : gmin;
+        gmax = bg > gmax ? bg : gmax;
+      }
     }
   }
   // wave reduction
@@ -220,13 +266,13 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
   LdmJob& J = jobs[blockIdx.x];
   LdmAxes& A = axes[blockIdx.x];
   __shared__ double wgax[LDM_MAX_AX], wlax[LDM_MAX_AX];
-  __shared__ double s_wg_min, s_wl_min;
+  __shared__ double s_wg_min, s_wl_min, s_wg_max, s_wl_max;
   __shared__ int s_ng, s_nl, s_status;
   __shared__ unsigned long long s_ext[4];
   {
-    // reduce the per-chunk width extremes written by k_axis
+    // reduce the per-item width extremes written by k_axis
     unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
-    const long long nch = (J.i1 - J.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK;
+    const long long nch = J.n_ax_part;
     for (long long c = threadIdx.x; c < nch; c += blockDim.x) {
       const unsigned long long* P = J.ax_part + 4 * c;
       lmin = P[0] < lmin ? P[0] : lmin;
@@ -258,9 +304,8 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
   }
   if (threadIdx.x == 0) {
     int status = 0, ng = 0, nl = 0;
-    double wg_min = 0, wl_min = 0;
+    double wg_min = 0, wl_min = 0, wg_max = 0, wl_max = 0;
     if (J.i1 > J.i0) {
-      double wg_max;
       if (J.single_iso) {
         wg_min = fwhm_gauss(db.nu0[J.i0], J.dopp[1]);
         wg_max = fwhm_gauss(db.nu0[J.i1 - 1], J.dopp[1]);
@@ -269,7 +314,7 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
         wg_max = bits_to_double(s_ext[3]);
       }
       wl_min = bits_to_double(s_ext[0]);
-      const double wl_max = bits_to_double(s_ext[1]);
+      wl_max = bits_to_double(s_ext[1]);
       if (!(wg_min > 0.0) || !(wl_min > 0.0)) {
         status |= LDM_ST_WIDTH;
       } else {
@@ -283,6 +328,7 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
     if (status) { ng = 0; nl = 0; }
     s_ng = ng; s_nl = nl; s_status = status;
     s_wg_min = wg_min; s_wl_min = wl_min;
+    s_wg_max = wg_max; s_wl_max = wl_max;
     A.ng = ng;
     A.nl = nl;
     A.m = ng * nl;
@@ -334,6 +380,10 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
     res[blockIdx.x].ng = A.ng;
     res[blockIdx.x].nl = A.nl;
     res[blockIdx.x].status = J.status;
+    res[blockIdx.x].wg_min = s_wg_min;
+    res[blockIdx.x].wg_max = s_wg_max;
+    res[blockIdx.x].wl_min = s_wl_min;
+    res[blockIdx.x].wl_max = s_wl_max;
   }
 }
 

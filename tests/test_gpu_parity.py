@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import ldm_oracle as oracle
-from radis_amd import engine, hitran, workloads
+from radis_amd import _native, engine, hitran, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -264,4 +264,48 @@ def test_min_rms_weights_match_oracle(dev, label, cond):
               f"min-RMS vs simple {oracle.rel_peak_error(simple, ref):.2e}")
         assert err <= REL_TOL
     b.close()
+    db.close()
+
+
+def _axis_lines(n, seed, multi_iso, with_nself):
+    import dataclasses
+
+    lines = workloads.co2_like(n, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    kw = {}
+    if multi_iso:
+        kw["iso"] = rng.integers(1, 4, n).astype(np.uint8)
+        kw["molar_mass"] = np.array([0.0, 44.0, 45.0, 46.0])
+    if with_nself:
+        kw["n_self"] = rng.uniform(0.4, 0.9, n).astype(np.float32)
+    return dataclasses.replace(lines, **kw)
+
+
+@pytest.mark.parametrize("multi_iso,with_nself", [(False, False), (True, True)])
+def test_axis_candidates_equal_full_scan(dev, multi_iso, with_nself):
+    """The width extremes from the per-block candidates are bit-identical to
+    a scan of every window line (RADIS_LDM_SCAN_AXIS), for T above, at and
+    below Tref and mole fractions 0, 0.3 and 1; both agree with NumPy's
+    FP64 widths to rounding."""
+    lines = _axis_lines(400_000, 40 + int(multi_iso), multi_iso, with_nself)
+    db = engine.LineDatabase(dev, lines)
+    for T, x in [(1500.0, 0.3), (296.0, 0.3), (200.0, 0.0), (900.0, 1.0)]:
+        c = workloads.Conditions(2003.7, 2371.2, 0.002, T, 1.3, x)
+        r0 = engine.request_from_conditions(db, c)
+        r1 = engine.request_from_conditions(db, c)
+        r1.flags = int(r1.flags) | _native.SCAN_AXIS
+        b = engine.SpectrumBatch(db, [r0, r1])
+        res = b.results(b.run())
+        i0, i1 = b.info(0), b.info(1)
+        for k in ("wg_min", "wg_max", "wl_min", "wl_max"):
+            assert getattr(i0, k) == getattr(i1, k), (T, x, k)
+        assert (i0.n_g, i0.n_l) == (i1.n_g, i1.n_l)
+        np.testing.assert_array_equal(res[0], res[1])
+        d = lines.as_dict()
+        sel = slice(np.searchsorted(d["nu0"], c.wmin, side="left"), np.searchsorted(d["nu0"], c.wmax, side="right"))
+        S, _, wG, wL = oracle.line_parameters({k: (v[sel] if isinstance(v, np.ndarray) and v.shape == d["nu0"].shape
+                                                   else v) for k, v in d.items()}, T, 1.3, x)
+        np.testing.assert_allclose([i0.wl_min, i0.wl_max], [wL.min(), wL.max()], rtol=1e-15)
+        np.testing.assert_allclose([i0.wg_min, i0.wg_max], [wG.min(), wG.max()], rtol=1e-15)
+        b.close()
     db.close()
