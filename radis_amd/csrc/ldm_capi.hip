@@ -573,6 +573,35 @@ void free_layout(radis_ldm_batch* b) {
 }
 
 // Allocate workspace, twiddle tables and work-item lists for every request.
+// Workspace of one request (tile width, pass splits and the offsets of its
+// buffers from `total`, which it advances); shared by the batch layout and
+// radis_ldm_request_bytes.
+int layout_request(Req& R, size_t& total) {
+  R.tile_w = R.r.tile_w ? R.r.tile_w : choose_tile(R.m_cap);
+  if (fill_tile_bytes(R.m_cap, R.tile_w) > (size_t)FILL_LDS_MAX)
+    return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit a fill tile of width %d (max %d rows at width %d)",
+                R.m_cap, R.tile_w, (int)(FILL_LDS_MAX / fill_tile_bytes(1, MIN_TILE_W)), MIN_TILE_W);
+  const int64_t V = R.nv + 1;
+  R.n_tiles = (int)((V + R.tile_w - 1) / R.tile_w);
+  R.stride = (int64_t)align_up((size_t)V, 64);
+  const int n1 = R.plan.n1, n2 = R.plan.n2;
+  if (R.plan.fc) {
+    R.bA = floor_log2(fc_cols_a(n1));
+    R.bB = floor_log2(fc_rows_b(n2));
+  } else {
+    R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
+    R.bB = std::min(floor_log2((n2 <= LDM_CAP_B ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
+  }
+  const int pcap = (R.m_cap + 1) / 2;
+  R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
+  R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
+  R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
+  R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
+  // axis partials: at most one per chunk, plus the two partial blocks at the window edges
+  R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2) * 32, 256);
+  return 0;
+}
+
 int build_layout(radis_ldm_batch* b) {
   free_layout(b);
   b->tiles_valid = false;
@@ -580,29 +609,9 @@ int build_layout(radis_ldm_batch* b) {
   size_t total = 0;
   size_t fill_lds = 0;
   for (Req& R : b->reqs) {
-    R.tile_w = R.r.tile_w ? R.r.tile_w : choose_tile(R.m_cap);
-    if (fill_tile_bytes(R.m_cap, R.tile_w) > (size_t)FILL_LDS_MAX)
-      return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit a fill tile of width %d (max %d rows at width %d)",
-                  R.m_cap, R.tile_w, (int)(FILL_LDS_MAX / fill_tile_bytes(1, MIN_TILE_W)), MIN_TILE_W);
-    const int64_t V = R.nv + 1;
-    R.n_tiles = (int)((V + R.tile_w - 1) / R.tile_w);
-    R.stride = (int64_t)align_up((size_t)V, 64);
+    int rc = layout_request(R, total);
+    if (rc) return rc;
     fill_lds = std::max(fill_lds, fill_tile_bytes(R.m_cap, R.tile_w));
-    const int n1 = R.plan.n1, n2 = R.plan.n2;
-    if (R.plan.fc) {
-      R.bA = floor_log2(fc_cols_a(n1));
-      R.bB = floor_log2(fc_rows_b(n2));
-    } else {
-      R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
-      R.bB = std::min(floor_log2((n2 <= LDM_CAP_B ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
-    }
-    const int pcap = (R.m_cap + 1) / 2;
-    R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
-    R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
-    R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
-    R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
-    // axis partials: at most one per chunk, plus the two partial blocks at the window edges
-    R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2) * 32, 256);
   }
   // --- twiddle tables, deduplicated by size
   std::map<int, size_t> sub_off;
@@ -1182,12 +1191,11 @@ int radis_ldm_request_bytes(const radis_ldm_db* db, const radis_ldm_request* req
   int rc = plan_one(*db, *req, R);
   if (rc) return rc;
   R.m_cap = R.m_need;
-  R.tile_w = req->tile_w ? req->tile_w : choose_tile(R.m_cap);
-  const int64_t V = R.nv + 1;
-  const int64_t stride = (int64_t)align_up((size_t)V, 64);
-  const int64_t tiles = (V + R.tile_w - 1) / R.tile_w;
-  const int64_t pcap = (R.m_cap + 1) / 2;
-  int64_t b = (int64_t)R.m_cap * stride * 4 + pcap * R.plan.L * 8 + R.plan.L * 8 + tiles * 16 + (int64_t)R.m_cap * 16;
+  size_t total = 0;
+  rc = layout_request(R, total);
+  if (rc) return rc;
+  // the batch's work items and job table (16 B per item, a few per tile and block)
+  int64_t b = (int64_t)total + (int64_t)R.n_tiles * 16;
   // twiddle tables, plus alignment slack: an upper bound of what a batch of
   // this one request allocates
   b += (int64_t)(R.plan.n1 + R.plan.n2 + 1024 + (R.plan.L >> LDM_TW_LO_BITS) + 1) * 8 + 64 * 1024;
