@@ -300,17 +300,87 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       xrs[s][q] = (float)(L / 2 - kk) * J.inv_Lws;
     }
   }
+  // Lineshape FTs of this thread's bins.  F_r(x) = exp(-a_l x^2) exp(-b_m x)
+  // for row r = l*N_L + m (plus alias terms where the row folds).  The
+  // Gaussian factor depends on l only and is kept per bin while consecutive
+  // rows share l (refilled N_G times).  The Lorentzian factor is geometric
+  // along a thread's bins: for fixed s the RL bins are kk = kk0 + i*L/RL in
+  // two runs of RL/2 (below and above L/2, folded), so exp(-b x) takes one
+  // exponential per run and a product per bin.  That is about 3 exponentials
+  // per row and thread instead of RL*ITL.
+  const int NLr = axes[it.x].nl;
+  const float dlt = (float)(L / SH::RL) * J.inv_Lws;
+  float u0[SH::ITL][2];
+#pragma unroll
+  for (int s = 0; s < SH::ITL; ++s) {
+    const int t = threadIdx.x + s * T;
+    const long long ks = (long long)(k1_0 + (t & (B - 1))) + (long long)N1 * (t >> LB);  // bin of q = 0
+    u0[s][0] = (float)ks * J.inv_Lws;                    // q = 0, 1, ..., RL/2-1
+    u0[s][1] = (float)(L / SH::RL - ks) * J.inv_Lws;     // q = RL-1, RL-2, ..., RL/2
+  }
+  float gcache[SH::ITL][SH::RL];
+  int g_l = -1;
+  auto gauss_fill = [&](int r) {
+    const float a = J.row_a[r];
+#pragma unroll
+    for (int s = 0; s < SH::ITL; ++s)
+#pragma unroll
+      for (int q = 0; q < SH::RL; ++q) gcache[s][q] = __expf(-(a * xs[s][q] * xs[s][q]));
+  };
+  auto row_ft = [&](int r, float (&F)[SH::ITL][SH::RL]) {
+    const float bco = J.row_b[r];
+    const float E = __expf(-bco * dlt);
+#pragma unroll
+    for (int s = 0; s < SH::ITL; ++s)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float H = __expf(-bco * u0[s][h]);
+#pragma unroll
+        for (int i = 0; i < SH::RL / 2; ++i) {
+          const int q = h ? SH::RL - 1 - i : i;
+          F[s][q] = gcache[s][q] * H;
+          H *= E;
+        }
+      }
+    const int nf = J.nfold[r];
+    if (nf > 0) {
+      const float a = J.row_a[r];
+#pragma unroll
+      for (int s = 0; s < SH::ITL; ++s)
+#pragma unroll
+        for (int q = 0; q < SH::RL; ++q)
+          for (int n = 1; n <= nf; ++n) {
+            const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xrs[s][q]) : ((float)n * J.half_inv_ws + xs[s][q]);
+            F[s][q] += __expf(-(a * xa * xa) - bco * xa);
+          }
+    }
+    const float f0 = J.row_inv_f0[r];
+#pragma unroll
+    for (int s = 0; s < SH::ITL; ++s)
+#pragma unroll
+      for (int q = 0; q < SH::RL; ++q) F[s][q] *= f0;
+  };
   if (P > 0) prefetch(0);
   for (int p = 0; p < P; ++p) {
     // the previous pair's last stage has finished reading LDS
     if (p > 0) __syncthreads();
     const int ra = 2 * p, rb = 2 * p + 1;
+    float fa[SH::ITL][SH::RL], fb[SH::ITL][SH::RL];
+    if (ra / NLr != g_l) { g_l = ra / NLr; gauss_fill(ra); }
+    row_ft(ra, fa);
+    if (rb < M) {
+      if (rb / NLr != g_l) { g_l = rb / NLr; gauss_fill(rb); }
+      row_ft(rb, fb);
+    } else {
+#pragma unroll
+      for (int s = 0; s < SH::ITL; ++s)
+#pragma unroll
+        for (int q = 0; q < SH::RL; ++q) fb[s][q] = 0.0f;
+    }
     auto ld = [&](int s, int q, int) -> float2 { return pre[s][q]; };
     auto st = [&](int s, int q, int, float2 z) {
-      const float fa = lineshape_ft_x(J, ra, xs[s][q], xrs[s][q]);
-      const float fb = (rb < M) ? lineshape_ft_x(J, rb, xs[s][q], xrs[s][q]) : 0.0f;
-      acc[s][q].x += z.x * fa + z.y * fb;
-      acc[s][q].y += z.y * fa - z.x * fb;
+      acc[s][q].x += z.x * fa[s][q] + z.y * fb[s][q];
+      acc[s][q].y += z.y * fa[s][q] - z.x * fb[s][q];
     };
     auto next = [&] {
       if (p + 1 < P) prefetch(p + 1);
