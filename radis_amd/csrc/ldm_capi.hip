@@ -487,9 +487,15 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.c_boltz = -HC_K * (1 / r.Tgas - 1 / r.Tref);
   const double dens = r.mole_fraction * ((r.pressure_bar * 1e5) / (K_KB * r.Tgas)) * 1e-6;  // factory.py:908
   J.i_scale = std::ldexp(dens, -R.scale_log2);
+  J.i_fast = 1;
   for (int s = 0; s < LDM_MAX_ISO; ++s) {
     J.dopp[s] = (s < db.n_iso_slots && db.molar_mass[s] > 0) ? dopp_factor(r.Tgas, db.molar_mass[s]) : 0.0;
     J.qratio[s] = r.q_ratio[s];
+    const double qi = r.q_ratio[s] * J.i_scale;
+    J.qi_f[s] = (float)qi;
+    // S_ref (>= ~1e-45) times qi must stay in FP32 range before the
+    // Boltzmann power of two is applied
+    if ((db.iso_mask & (1u << s)) && !(qi > 0x1p-60 && qi < 0x1p+100)) J.i_fast = 0;
   }
   J.dxG = r.dxG;
   J.dxL = r.dxL;

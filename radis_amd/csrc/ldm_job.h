@@ -71,6 +71,8 @@ struct LdmJob {
   double i_scale;                // number density * 2^-scale_log2
   double dopp[LDM_MAX_ISO];      // sqrt((2 Na k_b T ln2)/M_iso)
   double qratio[LDM_MAX_ISO];    // Q(Tref)/Q(T)
+  float qi_f[LDM_MAX_ISO];       // qratio * i_scale in FP32 (used when i_fast)
+  int i_fast;                    // 1: every qi_f is well inside FP32's range
   double dxG, dxL, fold_thresh;
   double shift_margin;           // max|delta_air|*p_atm + 2*delta_w
   float c_stim_T, c_stim_R;      // -hc_k/T, -hc_k/Tref (stimulated emission, FP32)

@@ -60,6 +60,42 @@ __device__ __forceinline__ double grid_position(double x, double wmin, double de
   return slope * (x - wj) + (double)j;
 }
 
+// The same as (index, fraction) = (int(pos), pos - int(pos)) for the fill.
+// NumPy's (1/(w[j+1]-w[j]))*(x-w[j]) + j reaches j+1 only when (x-w[j]) is
+// within a few ulp of j's spacing below the step (the sum's half-ulp is
+// below 2e-9 for j < 2^24), so away from that the index is j and the
+// fraction, which only feeds FP32 weights, takes an FP32 reciprocal instead
+// of the FP64 division.
+__device__ __forceinline__ void grid_index(double x, double wmin, double delta, double inv_delta, int nv, int& k0,
+                                           float& tv) {
+#pragma clang fp contract(off)
+  const double wlast = wmin + (double)(nv - 1) * delta;
+  if (x < wmin) { k0 = 0; tv = 0.0f; return; }
+  if (!(x < wlast)) { k0 = nv - 1; tv = 0.0f; return; }
+  int j = (int)((x - wmin) * inv_delta);
+  j = j > nv - 2 ? nv - 2 : j;
+  double wj = wmin + (double)j * delta;
+  if (wj > x && j > 0) {
+    --j;
+    wj = wmin + (double)j * delta;
+  }
+  double wj1 = wmin + (double)(j + 1) * delta;
+  if (wj1 <= x && j < nv - 2) {
+    ++j;
+    wj = wj1;
+    wj1 = wmin + (double)(j + 1) * delta;
+  }
+  const double d = x - wj, dl = wj1 - wj;
+  if (d < dl * (1.0 - 1e-7)) {
+    k0 = j;
+    tv = (float)d * __builtin_amdgcn_rcpf((float)dl);
+    return;
+  }
+  const double pos = (1.0 / dl) * d + (double)j;
+  k0 = (int)pos;
+  tv = (float)(pos - (double)k0);
+}
+
 // Same rule on a small tabulated increasing axis la[0..n) with the slopes
 // sl[j] = 1/(la[j+1]-la[j]) precomputed (NumPy computes the same quotient).
 __device__ __forceinline__ double axis_position(double x, const double* la, const double* sl, int n) {
@@ -477,6 +513,15 @@ __device__ __forceinline__ float boltzmann_scale(double s, double x) {
   return (float)ldexp(s * (double)__expf((float)r), (int)n);
 }
 
+// Same with the prefactor s already in FP32 range (the power of two is
+// applied in FP32; results below FP32's range are negligible here).
+__device__ __forceinline__ float boltzmann_scale32(float s, double x) {
+#pragma clang fp contract(off)
+  const double n = rint(x * 1.4426950408889634);
+  const double r = (x - n * 0.6931471805599453) - n * 2.3190468138462996e-17;
+  return ldexpf(s * __expf((float)r), (int)n);
+}
+
 struct LineTerms {
   int k0;
   int l0, m0;
@@ -532,11 +577,13 @@ struct LineCols {
 // line's atomics to retire).
 struct IsoVals {
   float lgc;   // log(2*dopp_iso/c_CGS)
+  float qi;    // Q(Tref)/Q(T) * number density * 2^-scale_log2 (when in FP32 range)
   double q;    // Q(Tref)/Q(T)
 };
 
 __device__ __forceinline__ void iso_vals(const LdmJob& J, int iso, IsoVals& v) {
   v.lgc = J.lgc_f[iso];
+  v.qi = J.qi_f[iso];
   v.q = J.qratio[iso];
 }
 
@@ -548,7 +595,7 @@ struct LineConst {
   double c_boltz, i_scale;
   float c_stim_T, c_stim_R, log_r_f, pa_air_f, pa_self_f;
   float laG0_f, inv_dxG_f, laL0_f, inv_dxL_f;
-  int nv, fp64_widths, has_delta, min_rms;
+  int nv, fp64_widths, has_delta, min_rms, i_fast;
   double c_vib, c_rot, c_ref;
   const LdmJob* job;
   __device__ explicit LineConst(const LdmJob& J, const LdmDb& db)
@@ -556,7 +603,7 @@ struct LineConst {
         x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
         c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
         laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
-        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), c_vib(J.c_vib),
+        fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), i_fast(J.i_fast), c_vib(J.c_vib),
         c_rot(J.c_rot), c_ref(J.c_ref), job(&J) {}
 };
 
@@ -624,9 +671,7 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 #pragma clang fp contract(off)
     nu = nu0 + ((double)r.dl * C.p_atm);
   }
-  const double pos = grid_position(nu, C.wmin, C.delta_w, C.inv_delta_w, C.nv);
-  t.k0 = (int)pos;
-  t.tv = (float)(pos - (double)t.k0);
+  grid_index(nu, C.wmin, C.delta_w, C.inv_delta_w, C.nv, t.k0, t.tv);
   const int iso = r.iso;
   const float nuf = (float)nu0;
   // FP32 positions on the log axes
@@ -680,9 +725,10 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   } else {
     const float xT = C.c_stim_T * nuf, xR = C.c_stim_R * nuf;
     float stim;
-    if (xT < -0.1f && xR < -0.1f) stim = __fdividef(1.0f - __expf(xT), 1.0f - __expf(xR));
+    if (xT < -0.1f && xR < -0.1f) stim = (1.0f - __expf(xT)) * __builtin_amdgcn_rcpf(1.0f - __expf(xR));
     else stim = expm1f(xT) / expm1f(xR);
-    t.I = boltzmann_scale(s64, C.c_boltz * (double)r.El) * stim;
+    const double x = C.c_boltz * (double)r.El;
+    t.I = (C.i_fast ? boltzmann_scale32(r.S * iv.qi, x) : boltzmann_scale(s64, x)) * stim;
   }
 }
 
