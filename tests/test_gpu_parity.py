@@ -309,3 +309,51 @@ def test_axis_candidates_equal_full_scan(dev, multi_iso, with_nself):
         np.testing.assert_allclose([i0.wg_min, i0.wg_max], [wG.min(), wG.max()], rtol=1e-15)
         b.close()
     db.close()
+
+
+def test_grid_index_at_and_just_below_grid_points(dev):
+    """Lines on grid points, and 1..64 ulp and 1e-12..1e-6 steps below the
+    next one, where NumPy's (1/dw)*(x - w[j]) + j rounds up to j + 1: the
+    index is NumPy's on every line (the fill's fast fraction path hands
+    these to the exact FP64 division)."""
+    wmin, wmax, wstep = 2000.0, 2400.0, 0.001
+    w = oracle.wavenumber_grid(wmin, wmax, wstep)
+    rng = np.random.default_rng(77)
+    j = rng.integers(1, w.size - 2, 600)
+    nu = [w[j]]
+    for k in (1, 2, 8, 64):
+        x = w[j + 1].copy()
+        for _ in range(k):
+            x = np.nextafter(x, -np.inf)
+        nu.append(x)
+    for frac in (1e-12, 1e-9, 1e-7, 1e-6):
+        nu.append(w[j + 1] - frac * wstep)
+    nu = np.sort(np.concatenate(nu))
+    n = nu.size
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    lines = workloads.LineList(nu0=nu, S_ref=f32(np.full(n, 1e-20)), El=f32(np.zeros(n)),
+                               g_air=f32(rng.uniform(0.03, 0.08, n)), g_self=f32(rng.uniform(0.05, 0.1, n)),
+                               n_air=f32(np.full(n, 0.7)), molar_mass=np.array([0.0, 28.0]), q_coef=(0.36, 1.0),
+                               name="grid_edges")
+    c = workloads.Conditions(wmin, wmax, wstep, 1000.0, 1.0, 0.1)
+    db, b, res = _run(dev, lines, [c])
+    inf = b.info(0)
+    ref, det = _oracle(lines, c, int(inf.fft_len), details=True)
+    _check_indices(b, det)
+    assert oracle.rel_peak_error(res[0], ref) <= REL_TOL
+    # NumPy's sum rounds up to j + 1 only where x's spacing is finer than
+    # half an ulp of j relative to the step (a grid starting near zero):
+    # w = 0.001 + 0.01 i, x one ulp below w[1] gets index 1
+    w2 = oracle.wavenumber_grid(0.001, 0.5, 0.01)
+    x = np.nextafter(w2[1], -np.inf)
+    assert oracle.get_indices(np.array([x]), w2)[0][0] == 1
+    nu2 = np.array([w2[0], x, w2[3] + 0.004])
+    lines2 = workloads.LineList(nu0=nu2, S_ref=f32(np.full(3, 1e-20)), El=f32(np.zeros(3)), g_air=f32(np.full(3, 0.05)),
+                                g_self=f32(np.full(3, 0.07)), n_air=f32(np.full(3, 0.7)),
+                                molar_mass=np.array([0.0, 28.0]), q_coef=(0.36, 1.0), name="round_up")
+    c2 = workloads.Conditions(0.001, 0.5, 0.01, 1000.0, 1.0, 0.1, dxG=0.5, dxL=0.5)
+    db2, b2, res2 = _run(dev, lines2, [c2])
+    ref2, det2 = _oracle(lines2, c2, int(b2.info(0).fft_len), details=True)
+    t2 = _check_indices(b2, det2)
+    assert t2["k0"][1] == 1 and t2["tv"][1] == 0.0
+    assert oracle.rel_peak_error(res2[0], ref2) <= REL_TOL
