@@ -233,6 +233,19 @@ int radis_ldm_batch_copy_ldm(radis_ldm_batch *b, int i, float *dst, void *stream
 const char *radis_ldm_last_error(void);
 int radis_ldm_abi_version(void);
 
+/* --- instrumental slit ------------------------------------------------------ */
+/* Convolution with a slit function sampled on the spectrum grid, on the
+ * device: out[i] = scale * sum_j slit[j] * in[i + (k-1)/2 - j], with in[]
+ * zero outside [0, n) -- scipy.signal.oaconvolve(in, slit, mode='same') *
+ * wstep as in convolve_with_slit (radis/tools/slit.py:477-757; pass the grid
+ * step as scale).  in, slit and out are device arrays of n, k and n floats;
+ * in and out must not overlap.  The slit is built and cropped by the caller
+ * ('valid' mode: radis/tools/slit.py:1005-1092).  Replaces the host
+ * convolution behind Spectrum.apply_slit (radis/spectrum/spectrum.py:3127),
+ * which the reference's GPU iterate leaves undone (radis/gpu/gpu.py:308-311). */
+int radis_ldm_slit_convolve(const float *in, int64_t n, const float *slit, int32_t k, double scale, float *out,
+                            void *stream);
+
 #ifdef __cplusplus
 }
 #endif

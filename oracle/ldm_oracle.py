@@ -34,6 +34,8 @@ FFT apply              lbl/broadening.py:2523-2530
 density, abscoeff      lbl/factory.py:908-918
 derived quantities     lbl/factory.py:1289-1297, spectrum/equations.py:21-44,
                        phys/blackbody.py:35-74, phys/convert.py:120-122
+slit convolution       tools/slit.py:735-737 (oaconvolve 'same' x wstep),
+                       1064-1071 (remove_boundary, 'valid')
 =====================  ==========================================================
 
 The one generalisation: the FFT length ``L`` is a parameter.  RADIS uses
@@ -448,6 +450,27 @@ def derived_quantities(wavenumber, abscoeff, path_length, Tgas):
     transmittance = 1 - emissivity
     radiance = emissivity * planck(cm2nm(np.asarray(wavenumber, dtype=np.float64)), Tgas)
     return dict(absorbance=absorbance, emissivity=emissivity, transmittance=transmittance, radiance=radiance)
+
+
+def slit_convolve(I, I_slit, wstep, mode="valid"):
+    """Convolution with a slit sampled on the spectrum grid and the 'valid'
+    boundary, as convolve_with_slit does after resampling
+    (tools/slit.py:735-737 ``oaconvolve(I, I_slit, 'same') * wstep``;
+    remove_boundary 1064-1071), written as a direct sum: out[i] =
+    wstep * sum_j I_slit[j] * I[i + (k-1)//2 - j]."""
+    I = np.asarray(I, dtype=np.float64)
+    I_slit = np.asarray(I_slit, dtype=np.float64)
+    n, k = I.size, I_slit.size
+    full = np.convolve(I, I_slit, mode="full")
+    c = (k - 1) // 2
+    out = full[c:c + n] * wstep
+    if mode == "valid":
+        la = min(n, k)
+        a, b = int((la - 1) / 2), int(la / 2)
+        out[:a] = np.nan
+        if b:
+            out[-b:] = np.nan
+    return out
 
 
 def residual_L2(a, b):

@@ -150,6 +150,7 @@ _PROTOTYPES = {
     "radis_ldm_batch_stage_ms": (c_int, [c_void_p, POINTER(c_float)]),
     "radis_ldm_batch_line_terms": (c_int, [c_void_p, c_int] + [c_void_p] * 7 + [c_void_p]),
     "radis_ldm_batch_copy_ldm": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "radis_ldm_slit_convolve": (c_int, [c_void_p, c_int64, c_void_p, c_int32, c_double, c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(_PROTOTYPES)

@@ -19,6 +19,7 @@
 #include "ldm_job.h"
 #include "ldm_kernels.h"
 #include "ldm_fftc.h"
+#include "ldm_slit.h"
 
 namespace {
 
@@ -1464,6 +1465,21 @@ int radis_ldm_batch_copy_ldm(radis_ldm_batch* b, int i, float* dst, void* stream
   const size_t cols = (size_t)(R.nv + 1);
   HIPCHK(hipMemcpy2DAsync(dst, cols * 4, (char*)b->work + R.off_ldm, (size_t)R.stride * 4, cols * 4, rows,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+int radis_ldm_slit_convolve(const float* in, int64_t n, const float* slit, int32_t k, double scale, float* out,
+                            void* stream) {
+  if (!in || !slit || !out) return fail(RADIS_LDM_EINVAL, "null pointer");
+  if (n <= 0 || k <= 0) return fail(RADIS_LDM_EINVAL, "need n > 0 and k > 0 (n=%lld, k=%d)", (long long)n, k);
+  if (!std::isfinite(scale)) return fail(RADIS_LDM_EINVAL, "non-finite scale");
+  const char *a0 = (const char*)in, *a1 = a0 + n * 4, *b0 = (const char*)out, *b1 = b0 + n * 4;
+  if (a0 < b1 && b0 < a1) return fail(RADIS_LDM_EINVAL, "input and output overlap");
+  const int64_t grid = (n + SLIT_OUT - 1) / SLIT_OUT;
+  if (grid > INT32_MAX) return fail(RADIS_LDM_ERANGE, "grid too long");
+  hipLaunchKernelGGL(k_slit_conv, dim3((unsigned)grid), dim3(SLIT_T), 0, (hipStream_t)stream, in, (long long)n, slit, k,
+                     scale, out);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

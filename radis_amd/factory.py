@@ -26,7 +26,8 @@ class Spectrum:
     DERIVED_NAMES = {"absorbance": "absorbance", "emissivity": "emissivity_noslit",
                      "transmittance": "transmittance_noslit", "radiance": "radiance_noslit"}
     UNITS = {"abscoeff": "cm-1", "absorbance": "", "emissivity_noslit": "", "transmittance_noslit": "",
-             "radiance_noslit": "mW/cm2/sr/nm", "wavenumber": "cm-1"}
+             "radiance_noslit": "mW/cm2/sr/nm", "wavenumber": "cm-1", "radiance": "mW/cm2/sr/nm",
+             "transmittance": "", "emissivity": ""}
 
     def __init__(self, wavenumber, abscoeff, derived, conditions, gpu_app=None, name=None):
         self._w = wavenumber
@@ -46,6 +47,40 @@ class Spectrum:
 
     def get_vars(self):
         return list(self._q)
+
+    def apply_slit(self, slit_function, unit="cm-1", shape="triangular", center_wavespace=None, norm_by="area",
+                   mode="valid", store=True):
+        """Convolve every ``*_noslit`` quantity with the slit on the GPU
+        (radis/spectrum/spectrum.py:3127-3500, cm-1 slits): radiance,
+        transmittance and emissivity.  Returns the spectrum."""
+        import torch
+
+        from . import slit as _slit
+
+        w = self._w
+        wstep = float(np.abs(np.diff(w)).min())
+        if center_wavespace is None:
+            center_wavespace = w[len(w) // 2]
+        wslit, Islit = _slit.get_slit_function(slit_function, unit=unit, norm_by=norm_by, shape=shape,
+                                               center_wavespace=center_wavespace, return_unit="cm-1", wstep=wstep)
+        if mode == "valid" and abs(w[-1] - w[0]) < abs(wslit[-1] - wslit[0]):
+            raise AssertionError("slit wider than the spectral range: nothing would be left in mode='valid'")
+        on_grid = _slit.slit_on_grid(wslit, Islit, wstep)
+        conv = {}
+        for qns in [k for k in self._q if k.endswith("_noslit")]:
+            x = torch.as_tensor(np.ascontiguousarray(self._q[qns], dtype=np.float32), device="cuda")
+            conv[qns[: -len("_noslit")]] = _slit.convolve_device(x, on_grid, wstep, mode=mode).double().cpu().numpy()
+        if store:
+            self._q.update(conv)
+            self._slit = (wslit, Islit)
+            self.conditions.update(slit_function=slit_function, slit_unit=unit, slit_shape=shape, norm_by=norm_by)
+        return self
+
+    def get_slit(self):
+        """(w, I) of the last slit applied (radis/spectrum/spectrum.py get_slit)."""
+        if getattr(self, "_slit", None) is None:
+            raise KeyError("no slit applied")
+        return self._slit
 
     def get(self, var, wunit="cm-1", Iunit="default", copy=True):
         """(wavenumber, quantity); only cm-1 and the default units."""
