@@ -239,7 +239,7 @@ int radis_ldm_abi_version(void);
  * zero outside [0, n) -- scipy.signal.oaconvolve(in, slit, mode='same') *
  * wstep as in convolve_with_slit (radis/tools/slit.py:477-757; pass the grid
  * step as scale).  in, slit and out are device arrays of n, k and n floats;
- * in and out must not overlap.  The slit is built and cropped by the caller
+ * out must not overlap in or slit.  The slit is built and cropped by the caller
  * ('valid' mode: radis/tools/slit.py:1005-1092).  Replaces the host
  * convolution behind Spectrum.apply_slit (radis/spectrum/spectrum.py:3127),
  * which the reference's GPU iterate leaves undone (radis/gpu/gpu.py:308-311). */

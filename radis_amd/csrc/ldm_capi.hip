@@ -1474,7 +1474,8 @@ int radis_ldm_slit_convolve(const float* in, int64_t n, const float* slit, int32
   if (n <= 0 || k <= 0) return fail(RADIS_LDM_EINVAL, "need n > 0 and k > 0 (n=%lld, k=%d)", (long long)n, k);
   if (!std::isfinite(scale)) return fail(RADIS_LDM_EINVAL, "non-finite scale");
   const char *a0 = (const char*)in, *a1 = a0 + n * 4, *b0 = (const char*)out, *b1 = b0 + n * 4;
-  if (a0 < b1 && b0 < a1) return fail(RADIS_LDM_EINVAL, "input and output overlap");
+  const char *s0 = (const char*)slit, *s1 = s0 + (int64_t)k * 4;
+  if ((a0 < b1 && b0 < a1) || (s0 < b1 && b0 < s1)) return fail(RADIS_LDM_EINVAL, "output overlaps the input or the slit");
   const int64_t grid = (n + SLIT_OUT - 1) / SLIT_OUT;
   if (grid > INT32_MAX) return fail(RADIS_LDM_ERANGE, "grid too long");
   hipLaunchKernelGGL(k_slit_conv, dim3((unsigned)grid), dim3(SLIT_T), 0, (hipStream_t)stream, in, (long long)n, slit, k,
