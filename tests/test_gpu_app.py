@@ -92,3 +92,30 @@ def test_gpuapp_iterate_matches_oracle():
     assert app.derived == {}
     np.testing.assert_array_equal(a, a2)
     app.__del__()
+
+
+@pytest.mark.gpu
+def test_two_live_gpuapps_do_not_share_state():
+    """The reference keeps its parameters in module globals, so two live
+    gpuApps clobber each other (radis/gpu/params.py:66-86); here each owns
+    its database and batch."""
+    vmin, dv, Nv = 2050.0, 0.002, 50_001
+    vmax = vmin + (Nv - 1) * dv
+    _, args = _factory_inputs(vmin, vmax)
+    lines = _oracle_lines(args)
+    n = args[0].size
+    half = [a[::2] if isinstance(a, np.ndarray) and a.shape == (n,) else a for a in args]
+    half[5] = args[5][:, ::2]  # gamma_arr
+    app1 = gpu_app.gpuApp(vmin, Nv, dv, 0.14, 0.2, *args)
+    app2 = gpu_app.gpuApp(vmin, Nv, dv, 0.14, 0.2, *half)
+    a1, _ = app1.iterate(1.0, 1000.0, 0.1)
+    a2, _ = app2.iterate(0.5, 2000.0, 0.3)
+    a1b, _ = app1.iterate(1.0, 1000.0, 0.1)
+    np.testing.assert_array_equal(a1, a1b)
+    ref1 = oracle.eq_abscoeff(lines, vmin, vmax, dv, 1000.0, 1.0, 0.1, 0.14, 0.2, L=int(app1._batch.info(0).fft_len))
+    ref2 = oracle.eq_abscoeff(_oracle_lines(tuple(half)), vmin, vmax, dv, 2000.0, 0.5, 0.3, 0.14, 0.2,
+                              L=int(app2._batch.info(0).fft_len))
+    assert oracle.rel_peak_error(a1, ref1) <= 1e-4
+    assert oracle.rel_peak_error(a2, ref2) <= 1e-4
+    app1.__del__()
+    app2.__del__()
