@@ -505,7 +505,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.log_r_f = (float)J.log_r;
   J.pa_air_f = (float)(J.p_atm * J.x_air);
   J.pa_self_f = (float)(J.p_atm * J.x_self);
-  for (int s = 0; s < LDM_MAX_ISO; ++s) J.lgc_f[s] = J.dopp[s] > 0 ? (float)std::log(2 * J.dopp[s] / 2.99792458e10) : 0.0f;
+  for (int s = 0; s < LDM_MAX_ISO; ++s) J.lgc_f[s] = J.dopp[s] > 0 ? (float)std::log2(2 * J.dopp[s] / 2.99792458e10) : 0.0f;
   J.fp64_widths = (r.flags & RADIS_LDM_FP64_WIDTHS) ? 1 : 0;
   J.min_rms = (r.flags & RADIS_LDM_MIN_RMS) ? 1 : 0;
   J.noneq = (r.flags & RADIS_LDM_NONEQ) ? 1 : 0;
@@ -1014,6 +1014,8 @@ void block_skyline(const radis_ldm_lines* L, int64_t i0, int cnt, int d, std::ve
 int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_db** out) {
   if (!ctx || !L || !out) return fail(RADIS_LDM_EINVAL, "null argument");
   if (L->n <= 0) return fail(RADIS_LDM_EINVAL, "empty line list");
+  // the fill addresses line columns with 32-bit byte offsets (8-byte nu0)
+  if (L->n > ((int64_t)1 << 29)) return fail(RADIS_LDM_ERANGE, "at most 2^29 lines per database (got %lld)", (long long)L->n);
   if (!L->nu0 || !L->S_ref || !L->El || !L->g_air || !L->g_self || !L->n_air || !L->molar_mass)
     return fail(RADIS_LDM_EINVAL, "missing required column");
   if (L->n_iso_slots < 2 || L->n_iso_slots > RADIS_LDM_MAX_ISO)

@@ -78,8 +78,8 @@ struct LdmJob {
   float c_stim_T, c_stim_R;      // -hc_k/T, -hc_k/Tref (stimulated emission, FP32)
   float log_r_f;                 // log(Tref/T), FP32
   float pa_air_f, pa_self_f;     // p_atm*(1-x), p_atm*x, FP32
-  float lgc_f[LDM_MAX_ISO];      // log(2*dopp_iso/c_CGS): log FWHM_G = log(nu0) + lgc
-  float laG0_f, inv_dxG_f;       // first G-axis node (log) and 1/dxG, FP32 (set by k_finalize)
+  float lgc_f[LDM_MAX_ISO];      // log2(2*dopp_iso/c_CGS): log2 FWHM_G = log2(nu0) + lgc
+  float laG0_f, inv_dxG_f;       // first G-axis node (log2) and ln2/dxG, FP32 (set by k_finalize)
   float laL0_f, inv_dxL_f;       // same for the L axis (set by k_finalize)
   int fp64_widths;               // 1: log-width positions always in FP64
   int min_rms;                   // 1: min-RMS weights

@@ -384,10 +384,12 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
   for (int i = threadIdx.x; i + 1 < s_ng; i += blockDim.x) A.slG[i] = 1.0 / (A.laG[i + 1] - A.laG[i]);
   for (int i = threadIdx.x; i + 1 < s_nl; i += blockDim.x) A.slL[i] = 1.0 / (A.laL[i + 1] - A.laL[i]);
   if (threadIdx.x == 0 && s_ng > 0) {
-    J.laG0_f = (float)A.laG[0];
-    J.laL0_f = (float)A.laL[0];
-    J.inv_dxG_f = (float)(1.0 / J.dxG);
-    J.inv_dxL_f = (float)(1.0 / J.dxL);
+    // FP32 fast positions work in log2 units (the hardware logarithm)
+    const double LOG2E = 1.4426950408889634, LN2 = 0.6931471805599453;
+    J.laG0_f = (float)(A.laG[0] * LOG2E);
+    J.laL0_f = (float)(A.laL[0] * LOG2E);
+    J.inv_dxG_f = (float)(LN2 / J.dxG);
+    J.inv_dxL_f = (float)(LN2 / J.dxL);
   }
   __syncthreads();
   const int ng = s_ng, nl = s_nl;
@@ -546,28 +548,36 @@ struct LineCols {
   const float *S, *El, *ga, *gs, *na, *ns, *dl;
   const uint8_t* iso;
   const float *evl, *evu;
-  long long dmask, imask;
+  unsigned dmask, imask;
   __device__ explicit LineCols(const LdmDb& db)
       : nu0(db.nu0), S(db.S_ref), El(db.El), ga(db.g_air), gs(db.g_self), na(db.n_air),
         ns(db.n_self ? db.n_self : db.n_air), dl(db.delta ? db.delta : db.dflt),
         iso(db.iso ? db.iso : reinterpret_cast<const uint8_t*>(db.dflt + 1)), evl(db.evib_l), evu(db.evib_u),
-        dmask(db.delta ? -1LL : 0LL), imask(db.iso ? -1LL : 0LL) {}
+        dmask(db.delta ? ~0u : 0u), imask(db.iso ? ~0u : 0u) {}
+  // Line i's columns.  The upload caps a database at 2^29 lines, so every
+  // byte offset fits 32 bits: each load is a uniform base plus one 32-bit
+  // offset (no 64-bit address arithmetic per column).
+  template <typename T>
+  __device__ __forceinline__ static T at(const T* p, unsigned byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + byte_off);
+  }
   // NONEQ: also the vibrational energies (the request checked they exist)
   template <bool NONEQ = false>
-  __device__ __forceinline__ void load(long long i, LineRaw& r) const {
+  __device__ __forceinline__ void load(unsigned i, LineRaw& r) const {
+    const unsigned o4 = i * 4u;
     if (NONEQ) {
-      r.evl = evl[i];
-      r.evu = evu[i];
+      r.evl = at(evl, o4);
+      r.evu = at(evu, o4);
     }
-    r.iso = iso[i & imask];  // first: the per-isotopologue lookups wait on it alone
-    r.nu0 = nu0[i];
-    r.dl = dl[i & dmask];
-    r.ga = ga[i];
-    r.gs = gs[i];
-    r.na = na[i];
-    r.ns = ns[i];
-    r.S = S[i];
-    r.El = El[i];
+    r.iso = at(iso, i & imask);  // first: the per-isotopologue lookups wait on it alone
+    r.nu0 = at(nu0, i * 8u);
+    r.dl = at(dl, o4 & (dmask << 2));
+    r.ga = at(ga, o4);
+    r.gs = at(gs, o4);
+    r.na = at(na, o4);
+    r.ns = at(ns, o4);
+    r.S = at(S, o4);
+    r.El = at(El, o4);
   }
 };
 
@@ -576,7 +586,7 @@ struct LineCols {
 // atomics (an LDS table read would make every line wait for the previous
 // line's atomics to retire).
 struct IsoVals {
-  float lgc;   // log(2*dopp_iso/c_CGS)
+  float lgc;   // log2(2*dopp_iso/c_CGS)
   float qi;    // Q(Tref)/Q(T) * number density * 2^-scale_log2 (when in FP32 range)
   double q;    // Q(Tref)/Q(T)
 };
@@ -678,8 +688,12 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   const float ra = __expf(r.na * C.log_r_f);
   const float rs = (r.ns == r.na) ? ra : __expf(r.ns * C.log_r_f);
   const float wl_f = 2.0f * (ra * r.ga * C.pa_air_f + rs * r.gs * C.pa_self_f);  // as fwhm_lorentz_f32
-  const float pg = (__logf(nuf) + iv.lgc - C.laG0_f) * C.inv_dxG_f;
-  const float pl = (__logf(wl_f) - C.laL0_f) * C.inv_dxL_f;
+  // positions on the log axes from base-2 logarithms (v_log_f32; ~1 ulp,
+  // far inside the tie margin that sends a line to the exact FP64 path)
+  const float l2wg = __builtin_amdgcn_logf(nuf) + iv.lgc;  // log2 FWHM_G
+  const float l2wl = __builtin_amdgcn_logf(wl_f);          // log2 FWHM_L
+  const float pg = (l2wg - C.laG0_f) * C.inv_dxG_f;
+  const float pl = (l2wl - C.laL0_f) * C.inv_dxL_f;
   const bool exact_g = C.fp64_widths || near_node(pg, ng);
   const bool exact_l = C.fp64_widths || near_node(pl, nl);
   if (exact_g) {
@@ -702,7 +716,7 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   }
   if (t.l0 > ng - 2) { t.l0 = ng - 2; t.tg = 1.0f; }
   if (t.m0 > nl - 2) { t.m0 = nl - 2; t.tl = 1.0f; }
-  if (C.min_rms) min_rms_weights(*C.job, __logf(nuf) + iv.lgc, wl_f, t);
+  if (C.min_rms) min_rms_weights(*C.job, l2wg * 0.69314718055994531f, wl_f, t);
   const double s64 = (double)r.S * iv.q * C.i_scale;
   // (1 - exp(-c2 nu/T)) / (1 - exp(-c2 nu/Tref)), radis/lbl/base.py:2664-2674:
   // for |x| >= 0.1 the difference 1 - e^x loses under 4 bits, so the fast
@@ -760,22 +774,22 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   if (k_hi > J.v) k_hi = J.v;
   const LineConst C(J, db);
   const LineCols cols(db);
-  const long long lo = J.i0 + it.z, hi = J.i0 + it.w;  // written by k_tile_bounds
+  const unsigned lo = (unsigned)(J.i0 + it.z), hi = (unsigned)(J.i0 + it.w);  // written by k_tile_bounds
   // Thread t takes lines lo + t, lo + t + T, ...: every load instruction of
   // a wave reads 64 consecutive lines.  (Giving each lane its own run of
   // lines spreads a wave's loads over as many cache lines and pages as it has
   // lanes, which measured up to 10x slower on dense lists; collisions of
   // neighbouring lines on one LDS address cost far less.)
-  const long long hlast = hi - 1;
+  const unsigned hlast = hi - 1;
   // first two lines in flight before the LDS set-up (indices clamped to the
   // tile's last line, which is valid whenever the thread has a line)
   LineRaw n1, n2;
   IsoVals v1;
-  const long long first = lo + threadIdx.x;
+  const unsigned first = lo + threadIdx.x;
   if (first < hi) {
     cols.load<NONEQ>(first, n1);
     iso_vals(J, n1.iso, v1);
-    const long long l1 = first + LDM_FILL_T;
+    const unsigned l1 = first + LDM_FILL_T;
     cols.load<NONEQ>(l1 < hlast ? l1 : hlast, n2);
   }
   {
@@ -787,12 +801,12 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   __syncthreads();
   const int wlen = (int)(k_hi - k_lo);
   const int sg = nl * Wp;
-  for (long long i = first; i < hi; i += LDM_FILL_T) {
+  for (unsigned i = first; i < hi; i += LDM_FILL_T) {
     const LineRaw cur = n1;
     const IsoVals cv = v1;
     n1 = n2;
     iso_vals(J, n1.iso, v1);
-    const long long i2 = i + 2 * LDM_FILL_T;
+    const unsigned i2 = i + 2 * LDM_FILL_T;
     cols.load<NONEQ>(i2 < hlast ? i2 : hlast, n2);
     LineTerms t;
     line_terms<NONEQ>(C, cur, cv, tab, ng, nl, t);
@@ -1114,11 +1128,11 @@ __global__ __launch_bounds__(256) void k_line_terms(const LdmJob* __restrict__ j
     IsoVals iv;
     LineTerms t;
     if (J.noneq) {
-      cols.load<true>(J.i0 + q, r);
+      cols.load<true>((unsigned)(J.i0 + q), r);
       iso_vals(J, r.iso, iv);
       line_terms<true>(C, r, iv, tab, A.ng, A.nl, t);
     } else {
-      cols.load(J.i0 + q, r);
+      cols.load((unsigned)(J.i0 + q), r);
       iso_vals(J, r.iso, iv);
       line_terms(C, r, iv, tab, A.ng, A.nl, t);
     }
