@@ -145,7 +145,9 @@ int radis_ldm_ctx_set_budget(radis_ldm_ctx *ctx, int64_t bytes);
 int radis_ldm_ctx_bytes_in_use(const radis_ldm_ctx *ctx, int64_t *bytes);
 
 /* --- line database -------------------------------------------------------- */
-/* Replaces the database_d upload of gpuApp.__init__ (gpu.py:174-203). */
+/* Replaces the database_d upload of gpuApp.__init__ (gpu.py:174-203).
+ * At most 2^29 lines (536 M) per database; larger lists are split into
+ * several databases (RADIS's own chunking of the line list does the same). */
 int radis_ldm_db_upload(radis_ldm_ctx *ctx, const radis_ldm_lines *lines, radis_ldm_db **out);
 void radis_ldm_db_destroy(radis_ldm_db *db);
 int radis_ldm_db_size(const radis_ldm_db *db, int64_t *n_lines);
