@@ -217,6 +217,11 @@ struct radis_ldm_batch {
   LdmJob* jobs_d = nullptr;
   LdmAxes* axes_d = nullptr;
   LdmResult* res_h[2] = {nullptr, nullptr};
+  // the job-table bytes last copied to jobs_d: an unchanged table is not
+  // copied again (the device side rewrites only what k_finalize recomputes)
+  std::vector<char> jobs_last;
+  // res_d holds results of a run not yet copied to res_h[last_slot]
+  bool res_stale = false;
   LdmResult* res_d = nullptr;
   int next_slot = 0, last_slot = 0;
   void* work = nullptr;
@@ -611,6 +616,7 @@ int layout_request(Req& R, size_t& total) {
 
 int build_layout(radis_ldm_batch* b) {
   free_layout(b);
+  b->jobs_last.clear();
   b->tiles_valid = false;
   // --- workspace
   size_t total = 0;
@@ -772,12 +778,20 @@ int plan_one(const radis_ldm_db& db, const radis_ldm_request& rq, Req& R) {
 // Host-to-device upload of the job table (outside any graph: a captured
 // copy from pinned memory is not guaranteed to re-read the host buffer).
 int upload_jobs(radis_ldm_batch* b, hipStream_t s, int slot) {
-  HIPCHK(hipMemcpyAsync(b->jobs_d, b->jobs_h[slot], sizeof(LdmJob) * b->n, hipMemcpyHostToDevice, s));
+  const size_t bytes = sizeof(LdmJob) * b->n;
+  const char* h = (const char*)b->jobs_h[slot];
+  if (b->jobs_last.size() == bytes && std::memcmp(b->jobs_last.data(), h, bytes) == 0) return 0;
+  HIPCHK(hipMemcpyAsync(b->jobs_d, h, bytes, hipMemcpyHostToDevice, s));
+  b->jobs_last.assign(h, h + bytes);
   return 0;
 }
 
-int download_results(radis_ldm_batch* b, hipStream_t s, int slot) {
-  HIPCHK(hipMemcpyAsync(b->res_h[slot], b->res_d, sizeof(LdmResult) * b->n, hipMemcpyDeviceToHost, s));
+// Results (axis sizes, status, width extremes) are copied back when they are
+// asked for, after the runs have completed, rather than after every run.
+int fetch_results(radis_ldm_batch* b) {
+  if (!b->res_stale) return 0;
+  HIPCHK(hipMemcpy(b->res_h[b->last_slot], b->res_d, sizeof(LdmResult) * b->n, hipMemcpyDeviceToHost));
+  b->res_stale = false;
   return 0;
 }
 
@@ -882,7 +896,7 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
 int enqueue(radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
   int rc = upload_jobs(b, s, slot);
   if (!rc) rc = launch_kernels(b, s, timed, slot);
-  if (!rc) rc = download_results(b, s, slot);
+  if (!rc) b->res_stale = true;
   return rc;
 }
 
@@ -1363,8 +1377,7 @@ int radis_ldm_batch_graph_replay(radis_ldm_batch* b, void* stream) {
   rc = upload_jobs(b, s, slot);
   if (rc) return rc;
   HIPCHK(hipGraphLaunch(b->gexec, s));
-  rc = download_results(b, s, slot);
-  if (rc) return rc;
+  b->res_stale = true;
   b->slot_timed[slot] = false;
   HIPCHK(hipEventRecord(b->ev_done[slot], s));
   b->pending[slot] = true;
@@ -1389,6 +1402,8 @@ int radis_ldm_batch_info(radis_ldm_batch* b, int i, radis_ldm_request_info* info
   info->m_cap = R.m_cap;
   info->tile_w = R.tile_w;
   info->scale_log2 = R.scale_log2;
+  rc = fetch_results(b);
+  if (rc) return rc;
   const LdmResult& res = b->res_h[b->last_slot][i];
   info->n_g = res.ng;
   info->n_l = res.nl;
@@ -1462,6 +1477,8 @@ int radis_ldm_batch_copy_ldm(radis_ldm_batch* b, int i, float* dst, void* stream
   int rc = wait_pending(b);
   if (rc) return rc;
   const Req& R = b->reqs[i];
+  rc = fetch_results(b);
+  if (rc) return rc;
   const int rows = b->res_h[b->last_slot][i].ng * b->res_h[b->last_slot][i].nl;
   if (rows == 0) return 0;
   const size_t cols = (size_t)(R.nv + 1);

@@ -305,6 +305,10 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
   __shared__ double s_wg_min, s_wl_min, s_wg_max, s_wl_max;
   __shared__ int s_ng, s_nl, s_status;
   __shared__ unsigned long long s_ext[4];
+  // the status is this run's own (the job table is not re-copied when the
+  // host has nothing new for it); the reduction's barriers order this reset
+  // before the fold-count atomics below
+  if (threadIdx.x == 0) J.status = 0;
   {
     // reduce the per-item width extremes written by k_axis
     unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
