@@ -79,6 +79,7 @@ int floor_log2(long long x) {
 // and the last child to go releases it.
 struct radis_ldm_ctx {
   int device = 0;
+  int n_cu = 256;       // compute units (persistent pass A: one workgroup each)
   int64_t budget = 0;
   int64_t in_use = 0;   // bytes of live line lists and batch workspaces
   int refs = 0;
@@ -199,7 +200,7 @@ struct Req {
   int tile_w = 0, n_tiles = 0;
   int64_t stride = 0;
   int scale_log2 = 0;
-  int bA = 0, bB = 0;
+  int bA = 0, bB = 0, bC = 0;  // log2 of the pass A / C column blocks and pass B row blocks
   int n_axp = 0;       // axis partials (items of k_axis)
   size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
@@ -600,9 +601,11 @@ int layout_request(Req& R, size_t& total) {
   if (R.plan.fc) {
     R.bA = floor_log2(fc_cols_a(n1));
     R.bB = floor_log2(fc_rows_b(n2));
+    R.bC = floor_log2(fc_cols_c(n1));
   } else {
     R.bA = std::min({6, floor_log2(LDM_CAP / n1), ctz(n2)});
     R.bB = std::min(floor_log2((n2 <= LDM_CAP_B ? LDM_CAP_B : LDM_CAP) / n2), ctz(n1));
+    R.bC = R.bA;
   }
   const int pcap = (R.m_cap + 1) / 2;
   R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
@@ -714,13 +717,13 @@ int build_layout(radis_ldm_batch* b) {
     (void)nl;
     for (int t = 0; t < R.n_tiles; ++t) fi.push_back(make_int4(j, t, 0, 0));
     const int pcap = (R.m_cap + 1) / 2;
-    const int nA = R.plan.n2 >> R.bA;
+    const int nA = R.plan.n2 >> R.bA, nC = R.plan.n2 >> R.bC;
     const int ca_ = R.plan.fc ? R.plan.n1 : 0;
     const int cb_ = R.plan.fc ? R.plan.n2 : (R.plan.n2 <= LDM_CAP_B ? 0 : -1);
     for (int p = 0; p < pcap; ++p)
       for (int cb = 0; cb < nA; ++cb) ma[ca_].push_back(make_int4(j, p, cb, 0));
     for (int kb = 0; kb < (R.plan.n1 >> R.bB); ++kb) mb[cb_].push_back(make_int4(j, kb, 0, 0));
-    for (int cb = 0; cb < nA; ++cb) mc[ca_].push_back(make_int4(j, 0, cb, 0));
+    for (int cb = 0; cb < nC; ++cb) mc[ca_].push_back(make_int4(j, 0, cb, 0));
   }
   std::vector<int4> fa, fb, fc;
   auto flatten = [](std::map<int, std::vector<int4>>& m, std::vector<int4>& v,
@@ -848,8 +851,10 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
     const int4* it = items + G.first;
     const bool f = g == 0, l = g == ng_a - 1;
     const dim3 grid(G.count);
+    // compiled pass A is persistent: one workgroup per CU loops over the column blocks
+    const dim3 grid_p(std::min(G.count, b->ctx->n_cu));
 #define FC_A(N) \
-  case N: launch_part(b, timed, slot, 4, f, l, k_fft_a_c<N>, grid, dim3(FC_T), 0, s, cj, ca, it, G.count); break;
+  case N: launch_part(b, timed, slot, 4, f, l, k_fft_a_c<N>, grid_p, dim3(FC_T_A), 0, s, cj, ca, it, G.count); break;
     switch (G.cls) {
       FC_A(256) FC_A(320) FC_A(384) FC_A(512) FC_A(640) FC_A(768) FC_A(1024)
       default: launch_part(b, timed, slot, 4, f, l, k_fft_a, grid, dim3(LDM_FFT_T), 0, s, cj, ca, it, G.count);
@@ -950,8 +955,11 @@ int radis_ldm_ctx_create(int device, radis_ldm_ctx** out) {
   HIPCHK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(RADIS_LDM_EINVAL, "device %d not in [0, %d)", device, n);
   HIPCHK(hipSetDevice(device));
+  int n_cu = 0;
+  HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
   auto* c = new radis_ldm_ctx();
   c->device = device;
+  if (n_cu > 0) c->n_cu = n_cu;
   *out = c;
   return 0;
 }

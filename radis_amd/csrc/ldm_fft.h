@@ -18,6 +18,40 @@
 #include <hip/hip_runtime.h>
 #include "ldm_job.h"
 
+// Global-memory (address space 1) view of a device pointer.  Pointers read
+// from the job table are generic to the compiler, and generic (flat) loads
+// and stores also count against the LDS wait counter: every LDS wait of an
+// FFT stage would then also wait for the global loads in flight.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glb(T* p) {
+  return (gptr<T>)p;
+}
+// element at a uniform base plus a 32-bit byte offset (float2 goes through
+// a native 2-vector: the float2 class's own copy and assignment operators
+// take generic pointers)
+typedef float gf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float gld(const float* p, unsigned byte_off) {
+  return *(gptr<const float>)((const char*)p + byte_off);
+}
+__device__ __forceinline__ float2 gld(const float2* p, unsigned byte_off) {
+  const gf2 v = *(gptr<const gf2>)((const char*)p + byte_off);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ int gld(const int* p, unsigned byte_off) {
+  return *(gptr<const int>)((const char*)p + byte_off);
+}
+__device__ __forceinline__ void gst(float* p, unsigned byte_off, float v) {
+  *(gptr<float>)((char*)p + byte_off) = v;
+}
+__device__ __forceinline__ void gst(float2* p, unsigned byte_off, float2 v) {
+  gf2 w;
+  w.x = v.x;
+  w.y = v.y;
+  *(gptr<gf2>)((char*)p + byte_off) = w;
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

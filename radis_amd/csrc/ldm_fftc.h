@@ -54,8 +54,18 @@ constexpr int pad_c(int c) { return c + ((c >> 5) << 1); }
 // st(s, q, e, value) instead of LDS (the last stage; no trailing barrier).
 // (s, q) = (thread pass, butterfly leg), the same for a given e in every
 // call, so callers can keep per-element state in registers indexed by them.
+// threadIdx.x behind an empty asm barrier: the stages' per-thread LDS and
+// twiddle addresses are then computed where they are used, instead of being
+// hoisted out of a caller's loop and held in registers across it
+__device__ __forceinline__ int fc_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST>
 __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st) {
+  const int tid = fc_tid();
   constexpr int NR = N / R;
   constexpr int ITEMS = NR * B;
   constexpr int IT = (ITEMS + T - 1) / T;
@@ -68,7 +78,7 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
   float2 v[IT][R];
 #pragma unroll
   for (int s = 0; s < IT; ++s) {
-    const int it = threadIdx.x + s * T;
+    const int it = tid + s * T;
     if (ITEMS % T == 0 || it < ITEMS) {
       if constexpr (IN_REG) {
 #pragma unroll
@@ -98,7 +108,7 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
   if constexpr (!IN_REG && !OUT_REG) __syncthreads();
 #pragma unroll
   for (int s = 0; s < IT; ++s) {
-    const int it = threadIdx.x + s * T;
+    const int it = tid + s * T;
     if (ITEMS % T == 0 || it < ITEMS) {
       const int j = it >> LB, b = it & (B - 1);
       const int k = j % NS;
@@ -126,18 +136,25 @@ struct FcNone {
 
 // Transform of the B columns in buf.  IN_REG: the first stage reads its
 // inputs through ld (then calls after_first()); OUT_REG: the last stage
-// hands its outputs to st.  A transform with both set touches LDS only
-// between its stages.
-template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, int S = 0, class LD, class ST, class AF>
+// hands its outputs to st (after before_last()).  A transform with both set
+// touches LDS only between its stages.
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, int S = 0, class LD, class ST, class AF, class BL>
 __device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
-                                          AF& after_first) {
+                                          AF& after_first, BL& before_last) {
   constexpr int NST = fc_nstages(N);
   if constexpr (S < NST) {
+    if constexpr (S == NST - 1) before_last();
     fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1>(buf, tw, ld,
                                                                                                         st);
     if constexpr (S == 0) after_first();
-    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, S + 1>(buf, tw, ld, st, after_first);
+    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, S + 1>(buf, tw, ld, st, after_first, before_last);
   }
+}
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST, class AF>
+__device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
+                                          AF& after_first) {
+  auto nop = [] {};
+  fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG>(buf, tw, ld, st, after_first, nop);
 }
 
 template <int N, int B, int T, bool FWD>
@@ -160,7 +177,7 @@ struct FcShape {
 template <int N, int T>
 __device__ __forceinline__ void fc_load_tw(float2* __restrict__ s_tw, const float2* __restrict__ tw) {
 #pragma unroll
-  for (int e = threadIdx.x; e < N; e += T) s_tw[e] = tw[e];
+  for (int e = threadIdx.x; e < N; e += T) s_tw[e] = gld(tw, 8u * (unsigned)e);
 }
 
 // Twiddles W_L^(m * (j0 + s*DS + q*DQ)) of one thread's last-stage outputs
@@ -172,8 +189,11 @@ template <int IT, int R>
 struct TwProg {
   float2 ws[IT], wq[R];
   __device__ __forceinline__ void init(const LdmJob& J, long long m, long long j0, long long DS, long long DQ) {
-    ws[0] = twiddle_L(J, m * j0);
-    const float2 A = twiddle_L(J, m * DS), C = twiddle_L(J, m * DQ);
+    set(twiddle_L(J, m * j0), twiddle_L(J, m * DS), twiddle_L(J, m * DQ));
+  }
+  // from the three table values W^(m j0), W^(m DS), W^(m DQ)
+  __device__ __forceinline__ void set(float2 base, float2 A, float2 C) {
+    ws[0] = base;
 #pragma unroll
     for (int s = 1; s < IT; ++s) ws[s] = cmul(ws[s - 1], A);
     wq[0] = make_float2(1.0f, 0.0f);
@@ -186,54 +206,109 @@ struct TwProg {
 // ---------------------------------------------------------------------------
 // Pass A with N1 fixed (see k_fft_a): B = fc_cols_a(N1) columns per workgroup.
 // ---------------------------------------------------------------------------
-#define FC_CAP 8192   // complex elements of a pass A / C tile (at most)
+// Pass A reads B consecutive floats of each of a column block's N1 rows
+// (stride N2): row segments of 128 B or more (B >= 32) stream at close to
+// the full HBM rate, 64 B ones at about 0.6 of it and 32 B ones at about
+// 0.35, so pass A takes the widest tile that fits LDS (one 1024-thread
+// workgroup per CU).
+#define FC_CAP_A 16384 // complex elements of a pass A tile (at most)
+#define FC_T_A 1024
+#define FC_CAP 8192    // complex elements of a pass C tile (at most)
 #define FC_T 512
-#define FC_CAP_B 2048 // complex elements of a pass B tile
+#define FC_CAP_B 2048  // complex elements of a pass B tile
 #define FC_T_B 256
-// columns per workgroup of pass A / C for N1, and rows per workgroup of pass B for N2
-constexpr int fc_cols_a(int n1) { return fc_pow2_floor(FC_CAP / n1); }
+// columns per workgroup of pass A and of pass C for N1, and rows per
+// workgroup of pass B for N2
+constexpr int fc_cols_a(int n1) { return fc_pow2_floor(FC_CAP_A / n1); }
+constexpr int fc_cols_c(int n1) { return fc_pow2_floor(FC_CAP / n1); }
 constexpr int fc_rows_b(int n2) { return FC_CAP_B / n2; }
 
 template <int N1>
-__global__ __launch_bounds__(FC_T, 4) void k_fft_a_c(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
-                                                     const int4* __restrict__ items, int n_items) {
-  constexpr int B = fc_cols_a(N1), LB = fc_log2(B), CAPE = N1 * B;
-  static_assert(CAPE % FC_T == 0, "tile is a whole number of thread passes");
+__global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
+                                                   const int4* __restrict__ items, int n_items) {
+  constexpr int B = fc_cols_a(N1), LB = fc_log2(B), CAPE = N1 * B, T = FC_T_A;
+  static_assert(CAPE % T == 0, "tile is a whole number of thread passes");
+  using SH = FcShape<N1, B, T>;
+  constexpr int NR0 = N1 / SH::R0;
+  constexpr int NSL = fc_ns(N1, fc_nstages(N1) - 1);
   __shared__ float2 buf[LdsSize<CAPE>::value];
   __shared__ float2 s_tw[N1];
-  const int4 it = items[xcd_block(blockIdx.x, n_items)];
-  const LdmJob& J = jobs[it.x];
-  const int p = it.y;
-  if (J.status || p >= axes[it.x].p) return;
-  const int N2 = J.n2;
-  const int n2_0 = it.z * B;
-  const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
-  const bool has_b = (2 * p + 1) < axes[it.x].m;
-  const float* rb = has_b ? ra + J.ldm_stride : ra;
-  const int V = (int)J.v;
-  // read by the second stage, after the first stage's barrier
-  fc_load_tw<N1, FC_T>(s_tw, J.tw1);
-  float2* Y = J.ybuf + (long long)p * J.L;
-  // output k1 of this thread's last-stage element (s, q): j0 + s*(T/B) + q*NS
-  using SH = FcShape<N1, B, FC_T>;
-  constexpr int NSL = fc_ns(N1, fc_nstages(N1) - 1);
-  TwProg<SH::ITL, SH::RL> tw;
-  tw.init(J, n2_0 + (threadIdx.x & (B - 1)), threadIdx.x >> LB, FC_T >> LB, NSL);
-  // first stage straight from the two LDM rows (zero beyond V), last stage
-  // straight to Y with the W_L twiddle: LDS holds only the middle stages
-  auto ld = [&](int, int, int e) -> float2 {
-    const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-    const bool in = n < V;
-    const int nc = in ? n : 0;
-    const float xa = ra[nc], xb = rb[nc];
-    return make_float2(in ? xa : 0.0f, (in && has_b) ? xb : 0.0f);
+  // Persistent: workgroup g takes the column blocks g, g + G, g + 2G, ...
+  // (G = gridDim.x <= n_items).  A block's first-stage inputs sit in v; the
+  // next block's loads go into v right after the first stage has consumed
+  // them, so they are in flight during the LDS stages and the stores.
+  float2 v[SH::IT0][SH::R0];
+  // issue the loads of block idx into v; false (for the whole workgroup)
+  // when the block has no work
+  auto fetch = [&](int idx) -> bool {
+    if (idx >= n_items) return false;
+    const int4 it = items[idx];
+    const LdmJob& J = jobs[it.x];
+    const int p = it.y;
+    if (J.status || p >= axes[it.x].p) return false;
+    const int N2 = J.n2, n2_0 = it.z * B, V = (int)J.v;
+    const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
+    const bool has_b = (2 * p + 1) < axes[it.x].m;
+    const float* rb = has_b ? ra + J.ldm_stride : ra;
+    // the two LDM rows, zero beyond V (and row b zero when the pair is odd)
+#pragma unroll
+    for (int s = 0; s < SH::IT0; ++s) {
+      const int it0 = threadIdx.x + s * T;
+#pragma unroll
+      for (int q = 0; q < SH::R0; ++q) {
+        const int e = it0 + q * NR0 * B;
+        const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
+        const bool in = (SH::ITEMS0 % T == 0 || it0 < SH::ITEMS0) && n < V;
+        const unsigned o = in ? 4u * (unsigned)n : 0u;
+        const float xa = gld(ra, o), xb = gld(rb, o);
+        v[s][q] = make_float2(in ? xa : 0.0f, (in && has_b) ? xb : 0.0f);
+      }
+    }
+    return true;
   };
-  auto st = [&](int s, int q, int e, float2 z) {
-    const int k1 = e >> LB, n2 = n2_0 + (e & (B - 1));
-    Y[(long long)k1 * N2 + n2] = cmul(z, tw(s, q));
-  };
-  auto nop = [] {};
-  fc_fft_io<N1, B, FC_T, true, true, true>(buf, s_tw, ld, st, nop);
+  // W_N1 table: the same for every request of this N1
+  fc_load_tw<N1, T>(s_tw, jobs[items[blockIdx.x].x].tw1);
+  bool live = fetch(blockIdx.x);
+  for (int idx = blockIdx.x; idx < n_items; idx += gridDim.x) {
+    if (!live) {
+      live = fetch(idx + gridDim.x);
+      continue;
+    }
+    const int4 it = items[idx];
+    const LdmJob& J = jobs[it.x];
+    const int N2 = J.n2, n2_0 = it.z * B;
+    float2* Y = J.ybuf + (long long)it.y * J.L;
+    // output k1 of this thread's last-stage element (s, q): j0 + s*(T/B) + q*NS;
+    // the table halves of the progression's three factors load now and are
+    // multiplied out just before the last stage
+    float2 twr[3][2];
+    {
+      const unsigned m = n2_0 + (threadIdx.x & (B - 1));
+      const unsigned ex[3] = {m * (threadIdx.x >> LB), m * (T >> LB), m * NSL};  // < L <= 2^21
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        twr[t][0] = gld(J.twl_hi, 8u * (ex[t] >> LDM_TW_LO_BITS));
+        twr[t][1] = gld(J.twl_lo, 8u * (ex[t] & ((1u << LDM_TW_LO_BITS) - 1)));
+      }
+    }
+    TwProg<SH::ITL, SH::RL> tw;
+    auto before_last = [&] {
+      tw.set(cmul(twr[0][0], twr[0][1]), cmul(twr[1][0], twr[1][1]), cmul(twr[2][0], twr[2][1]));
+    };
+    auto ld = [&](int s, int q, int) -> float2 { return v[s][q]; };
+    auto st = [&](int s, int q, int e, float2 z) {
+      const unsigned k1 = (unsigned)e >> LB, n2 = n2_0 + (e & (B - 1));
+      // byte offsets are below 8 L <= 2^24
+      gst(Y, 8u * (k1 * (unsigned)N2 + n2), cmul(z, tw(s, q)));
+    };
+    bool next = false;
+    auto prefetch = [&] { next = fetch(idx + gridDim.x); };
+    // first stage from v, last stage straight to Y with the W_L twiddle
+    fc_fft_io<N1, B, T, true, true, true>(buf, s_tw, ld, st, prefetch, before_last);
+    live = next;
+    // the last stage read LDS; the next block's first stage writes it
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -281,7 +356,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
 #pragma unroll
         for (int q = 0; q < SH::R0; ++q) {
           const int e = t + q * NR0 * B;
-          pre[s][q] = Yp[(e & (B - 1)) * N2 + (e >> LB)];
+          pre[s][q] = gld(Yp, 8u * (unsigned)((e & (B - 1)) * N2 + (e >> LB)));
         }
       }
     }
@@ -321,14 +396,14 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   float gcache[SH::ITL][SH::RL];
   int g_l = -1;
   auto gauss_fill = [&](int r) {
-    const float a = J.row_a[r];
+    const float a = gld(J.row_a, 4u * (unsigned)r);
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
       for (int q = 0; q < SH::RL; ++q) gcache[s][q] = __expf(-(a * xs[s][q] * xs[s][q]));
   };
   auto row_ft = [&](int r, float (&F)[SH::ITL][SH::RL]) {
-    const float bco = J.row_b[r];
+    const float bco = gld(J.row_b, 4u * (unsigned)r);
     const float E = __expf(-bco * dlt);
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s)
@@ -342,9 +417,9 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
           H *= E;
         }
       }
-    const int nf = J.nfold[r];
+    const int nf = gld(J.nfold, 4u * (unsigned)r);
     if (nf > 0) {
-      const float a = J.row_a[r];
+      const float a = gld(J.row_a, 4u * (unsigned)r);
 #pragma unroll
       for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
@@ -354,7 +429,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
             F[s][q] += __expf(-(a * xa * xa) - bco * xa);
           }
     }
-    const float f0 = J.row_inv_f0[r];
+    const float f0 = gld(J.row_inv_f0, 4u * (unsigned)r);
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
@@ -407,7 +482,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   tw.init(J, k1_0 + (threadIdx.x & (B - 1)), threadIdx.x >> LB, T >> LB, NSL);
   auto st_out = [&](int s, int q, int e, float2 z) {
     const int b = e & (B - 1), n2 = e >> LB;
-    Bo[b * N2 + n2] = cmulc(z, tw(s, q));
+    gst(Bo, 8u * (unsigned)(b * N2 + n2), cmulc(z, tw(s, q)));
   };
   FcNone none;
   auto nop = [] {};
@@ -421,7 +496,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
 template <int N1>
 __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                                      int n_items) {
-  constexpr int B = fc_cols_a(N1), LB = fc_log2(B), CAPE = N1 * B;
+  constexpr int B = fc_cols_c(N1), LB = fc_log2(B), CAPE = N1 * B;
   static_assert(CAPE % FC_T == 0, "tile is a whole number of thread passes");
   __shared__ float2 buf[LdsSize<CAPE>::value];
   __shared__ float2 s_tw[N1];
@@ -435,13 +510,15 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ 
     for (int s = 0; s < CAPE / FC_T; ++s) {
       const int e = threadIdx.x + s * FC_T;
       const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-      if (n < nv) J.out[n] = 0.0f;
+      if (n < nv) gst(J.out, 4u * (unsigned)n, 0.0f);
     }
     return;
   }
   fc_load_tw<N1, FC_T>(s_tw, J.tw1);
   const float sc = J.out_scale;
-  auto ld = [&](int, int, int e) -> float2 { return J.bbuf[(long long)(e >> LB) * N2 + n2_0 + (e & (B - 1))]; };
+  auto ld = [&](int, int, int e) -> float2 {
+    return gld(J.bbuf, 8u * (unsigned)((e >> LB) * N2 + n2_0 + (e & (B - 1))));  // < 8 L <= 2^24
+  };
   auto nop = [] {};
   if (J.derived) {  // uniform per workgroup: the plain store path stays as lean as without the epilogue
     auto st = [&](int, int, int e, float2 z) {
@@ -452,7 +529,7 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ 
   } else {
     auto st = [&](int, int, int e, float2 z) {
       const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
-      if (n < nv) J.out[n] = z.x * sc;
+      if (n < nv) gst(J.out, 4u * (unsigned)n, z.x * sc);
     };
     fc_fft_io<N1, B, FC_T, false, true, true>(buf, s_tw, ld, st, nop);
   }

@@ -874,8 +874,8 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
 
 // four-step twiddle W_L^e (forward sign)
 __device__ __forceinline__ float2 twiddle_L(const LdmJob& J, long long e) {
-  const float2 h = J.twl_hi[e >> LDM_TW_LO_BITS];
-  const float2 l = J.twl_lo[e & ((1 << LDM_TW_LO_BITS) - 1)];
+  const float2 h = gld(J.twl_hi, 8u * (unsigned)(e >> LDM_TW_LO_BITS));  // e < L <= 2^23
+  const float2 l = gld(J.twl_lo, 8u * (unsigned)(e & ((1 << LDM_TW_LO_BITS) - 1)));
   return cmul(h, l);
 }
 
