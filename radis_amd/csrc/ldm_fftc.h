@@ -63,9 +63,9 @@ __device__ __forceinline__ int fc_tid() {
   return t;
 }
 
-template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST>
+template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, class LD, class ST>
 __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st) {
-  const int tid = fc_tid();
+  const int tid = OPQ ? fc_tid() : (int)threadIdx.x;
   constexpr int NR = N / R;
   constexpr int ITEMS = NR * B;
   constexpr int IT = (ITEMS + T - 1) / T;
@@ -138,23 +138,26 @@ struct FcNone {
 // inputs through ld (then calls after_first()); OUT_REG: the last stage
 // hands its outputs to st (after before_last()).  A transform with both set
 // touches LDS only between its stages.
-template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, int S = 0, class LD, class ST, class AF, class BL>
+// OPQ: the stages take the thread index through fc_tid() (for callers that
+// run the transform inside a loop).
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int S = 0, class LD, class ST, class AF,
+          class BL>
 __device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
                                           AF& after_first, BL& before_last) {
   constexpr int NST = fc_nstages(N);
   if constexpr (S < NST) {
     if constexpr (S == NST - 1) before_last();
-    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1>(buf, tw, ld,
-                                                                                                        st);
+    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ>(buf, tw, ld,
+                                                                                                             st);
     if constexpr (S == 0) after_first();
-    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, S + 1>(buf, tw, ld, st, after_first, before_last);
+    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, OPQ, S + 1>(buf, tw, ld, st, after_first, before_last);
   }
 }
 template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST, class AF>
 __device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
                                           AF& after_first) {
   auto nop = [] {};
-  fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG>(buf, tw, ld, st, after_first, nop);
+  fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, false>(buf, tw, ld, st, after_first, nop);
 }
 
 template <int N, int B, int T, bool FWD>
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
     bool next = false;
     auto prefetch = [&] { next = fetch(idx + gridDim.x); };
     // first stage from v, last stage straight to Y with the W_L twiddle
-    fc_fft_io<N1, B, T, true, true, true>(buf, s_tw, ld, st, prefetch, before_last);
+    fc_fft_io<N1, B, T, true, true, true, true>(buf, s_tw, ld, st, prefetch, before_last);
     live = next;
     // the last stage read LDS; the next block's first stage writes it
     __syncthreads();
