@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RADIS_LDM_ABI_VERSION 3
+#define RADIS_LDM_ABI_VERSION 4
 
 enum {
     RADIS_LDM_OK = 0,
@@ -116,6 +116,11 @@ typedef struct {
 /* Validation: find the width extremes by scanning every window line instead
  * of the per-block candidates (same result, bit for bit). */
 #define RADIS_LDM_SCAN_AXIS 16
+/* Q(Tref)/Q(Tgas) of every isotopologue slot with lines from the database's
+ * partition-function splines (radis_ldm_db_set_qspline), evaluated on the
+ * device; q_ratio is then ignored for those slots.  Tgas and Tref must lie
+ * in each table's range.  Equilibrium requests only. */
+#define RADIS_LDM_DEVICE_Q 32
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {
@@ -151,6 +156,22 @@ int radis_ldm_ctx_bytes_in_use(const radis_ldm_ctx *ctx, int64_t *bytes);
 int radis_ldm_db_upload(radis_ldm_ctx *ctx, const radis_ldm_lines *lines, radis_ldm_db **out);
 void radis_ldm_db_destroy(radis_ldm_db *db);
 int radis_ldm_db_size(const radis_ldm_db *db, int64_t *n_lines);
+
+/* Tabulated partition function Q(T) of isotopologue slot iso as a B-spline
+ * in scipy's splrep form: n_knots knots (non-decreasing, at most 256),
+ * n_knots - degree - 1 coefficients, degree 1..5, valid for T in
+ * [T_lo, T_hi].  The reference builds the same spline from a Q(T) table
+ * (PartFuncCO2_CDSDtab: splrep, splev, 5 % range margin,
+ * radis/levels/partfunc_cdsd.py:84-117) and hands gpuApp host Q callables
+ * (radis/lbl/factory.py:1188-1199, radis/gpu/params.py:154-155); with a
+ * spline per slot, requests flagged RADIS_LDM_DEVICE_Q need no host Q.
+ * Synchronous; do not call while a run that reads the database is in
+ * flight. */
+int radis_ldm_db_set_qspline(radis_ldm_db *db, int iso, const double *knots, int32_t n_knots, const double *coefs,
+                             int32_t degree, double T_lo, double T_hi);
+/* Q(T) of slot iso at n temperatures: T and Q are device FP64 arrays
+ * (splev semantics, extrapolating past the knots).  Enqueued on stream. */
+int radis_ldm_qspline_eval(const radis_ldm_db *db, int iso, const double *T, int64_t n, double *Q, void *stream);
 
 /* --- planning helpers ----------------------------------------------------- */
 /* Grid length of arange(wmin, wmax+wstep, wstep) and the planner's FFT length. */

@@ -36,6 +36,9 @@ derived quantities     lbl/factory.py:1289-1297, spectrum/equations.py:21-44,
                        phys/blackbody.py:35-74, phys/convert.py:120-122
 slit convolution       tools/slit.py:735-737 (oaconvolve 'same' x wstep),
                        1064-1071 (remove_boundary, 'valid')
+tabulated Q(T)         levels/partfunc_cdsd.py:84-117 (splrep/splev of a Q(T)
+                       table; FITPACK's splev/fpbspl, scipy >= 1.4.0 per the
+                       reference's pyproject.toml)
 =====================  ==========================================================
 
 The one generalisation: the FFT length ``L`` is a parameter.  RADIS uses
@@ -471,6 +474,37 @@ def slit_convolve(I, I_slit, wstep, mode="valid"):
         if b:
             out[-b:] = np.nan
     return out
+
+
+def splev_fitpack(t, c, k, x):
+    """B-spline value as FITPACK's splev (ext=0) computes it, the evaluation
+    behind PartFuncCO2_CDSDtab.at (levels/partfunc_cdsd.py:105-117): the
+    interval l is the largest l in [k, n-k-2] with t[l] <= x (the end pieces
+    extrapolate), fpbspl's Cox-de Boor recursion gives the k+1 nonzero
+    B-splines h, and the value is sum_j c[l-k+j] h[j].  Pure Python loops
+    (small inputs)."""
+    t = np.asarray(t, dtype=np.float64)
+    c = np.asarray(c, dtype=np.float64)
+    n = t.size
+    out = []
+    for xv in np.atleast_1d(np.asarray(x, dtype=np.float64)):
+        l = k
+        while l < n - k - 2 and t[l + 1] <= xv:
+            l += 1
+        h = [1.0] + [0.0] * k
+        for j in range(1, k + 1):
+            hh = h[:j]
+            h[0] = 0.0
+            for i in range(1, j + 1):
+                li, lj = l + i, l + i - j
+                if t[li] == t[lj]:
+                    h[i] = 0.0
+                    continue
+                f = hh[i - 1] / (t[li] - t[lj])
+                h[i - 1] = h[i - 1] + f * (t[li] - xv)
+                h[i] = f * (xv - t[lj])
+        out.append(sum(c[l - k + j] * h[j] for j in range(k + 1)))
+    return np.asarray(out).reshape(np.shape(x))
 
 
 def residual_L2(a, b):

@@ -88,7 +88,8 @@ GENERIC_FFT = 2
 MIN_RMS = 4
 NONEQ = 8
 SCAN_AXIS = 16
-ABI_VERSION = 3
+DEVICE_Q = 32
+ABI_VERSION = 4
 
 
 class RequestInfo(ctypes.Structure):
@@ -135,6 +136,9 @@ _PROTOTYPES = {
     "radis_ldm_db_upload": (c_int, [c_void_p, POINTER(Lines), POINTER(c_void_p)]),
     "radis_ldm_db_destroy": (None, [c_void_p]),
     "radis_ldm_db_size": (c_int, [c_void_p, POINTER(c_int64)]),
+    "radis_ldm_db_set_qspline": (c_int, [c_void_p, c_int, POINTER(c_double), c_int32, POINTER(c_double), c_int32,
+                                         c_double, c_double]),
+    "radis_ldm_qspline_eval": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
     "radis_ldm_plan_grid": (c_int, [c_double, c_double, c_double, POINTER(c_int64), POINTER(c_int64)]),
     "radis_ldm_request_bytes": (c_int, [c_void_p, POINTER(Request), POINTER(c_int64), POINTER(RequestInfo)]),
     "radis_ldm_batch_create": (c_int, [c_void_p, c_void_p, c_int, POINTER(Request), POINTER(c_void_p)]),

@@ -111,6 +111,9 @@ struct radis_ldm_db {
   std::vector<float> dv_min, dv_max;  // Evib_u - Evib_l (two-temperature bound)
   bool has_sky = false;                // Lorentzian-width candidates per block (k_axis)
   std::vector<int> h_sky_off;          // host copy of the candidate offsets
+  // partition-function splines (radis_ldm_db_set_qspline): device slots and host copies
+  LdmQSpline* qs_mem = nullptr;
+  std::vector<LdmQSpline> h_qs;
 };
 
 namespace {
@@ -164,6 +167,10 @@ void db_free(radis_ldm_db* db) {
   if (db->mem) {
     (void)hipFree(db->mem);
     db->ctx->in_use -= (int64_t)db->bytes;
+  }
+  if (db->qs_mem) {
+    (void)hipFree(db->qs_mem);
+    db->ctx->in_use -= (int64_t)(sizeof(LdmQSpline) * LDM_MAX_ISO);
   }
   radis_ldm_ctx* ctx = db->ctx;
   delete db;
@@ -392,6 +399,20 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   const radis_ldm_request& r = R.r;
   if ((r.flags & RADIS_LDM_NONEQ) && !db.has_evib)
     return fail(RADIS_LDM_EINVAL, "RADIS_LDM_NONEQ needs a line list with Evib_l and Evib_u");
+  if (r.flags & RADIS_LDM_DEVICE_Q) {
+    if (r.flags & RADIS_LDM_NONEQ)
+      return fail(RADIS_LDM_EINVAL, "RADIS_LDM_DEVICE_Q takes equilibrium partition functions (not with RADIS_LDM_NONEQ)");
+    for (int s = 0; s < db.n_iso_slots; ++s) {
+      if (!(db.iso_mask & (1u << s))) continue;
+      if (!(db.dev.qs_mask & (1u << s)))
+        return fail(RADIS_LDM_EINVAL, "RADIS_LDM_DEVICE_Q: isotopologue slot %d has lines but no partition-function spline", s);
+      const LdmQSpline& q = db.h_qs[s];
+      // PartFuncCO2_CDSDtab._inrange, radis/levels/partfunc_cdsd.py:101-103
+      for (double T : {r.Tgas, r.Tref})
+        if (!(T >= q.lo && T <= q.hi))
+          return fail(RADIS_LDM_ERANGE, "temperature %g K outside the partition table of slot %d [%g, %g]", T, s, q.lo, q.hi);
+    }
+  }
   if (R.i1 <= R.i0) {
     R.m_need = 1;
     R.scale_log2 = 0;
@@ -436,8 +457,11 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   if (smax > 0) {
     const bool neq = (r.flags & RADIS_LDM_NONEQ) != 0;
     double qmax = 0;
+    const bool dq = (r.flags & RADIS_LDM_DEVICE_Q) != 0;
     for (int s = 0; s < db.n_iso_slots; ++s)
-      if (db.iso_mask & (1u << s)) qmax = std::max(qmax, r.q_ratio[s]);
+      if (db.iso_mask & (1u << s))
+        // device-evaluated ratios: the same spline evaluated here (for the bound only)
+        qmax = std::max(qmax, dq ? qspline_eval(db.h_qs[s], r.Tref) / qspline_eval(db.h_qs[s], r.Tgas) : r.q_ratio[s]);
     const double numin = std::max(db.h_nu0[R.i0], 1e-3);
     double boltz, stim;
     if (!neq) {
@@ -501,8 +525,10 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
     const double qi = r.q_ratio[s] * J.i_scale;
     J.qi_f[s] = (float)qi;
     // S_ref (>= ~1e-45) times qi must stay in FP32 range before the
-    // Boltzmann power of two is applied
-    if ((db.iso_mask & (1u << s)) && !(qi > 0x1p-60 && qi < 0x1p+100)) J.i_fast = 0;
+    // Boltzmann power of two is applied (k_finalize decides it for the
+    // slots whose ratio it computes)
+    const bool dev_q = (r.flags & RADIS_LDM_DEVICE_Q) && (db.dev.qs_mask & (1u << s));
+    if ((db.iso_mask & (1u << s)) && !dev_q && !(qi > 0x1p-60 && qi < 0x1p+100)) J.i_fast = 0;
   }
   J.dxG = r.dxG;
   J.dxL = r.dxL;
@@ -515,6 +541,8 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.fp64_widths = (r.flags & RADIS_LDM_FP64_WIDTHS) ? 1 : 0;
   J.min_rms = (r.flags & RADIS_LDM_MIN_RMS) ? 1 : 0;
   J.noneq = (r.flags & RADIS_LDM_NONEQ) ? 1 : 0;
+  J.device_q = (r.flags & RADIS_LDM_DEVICE_Q) ? 1 : 0;
+  J.Tref = r.Tref;
   J.c_vib = J.noneq ? -HC_K / r.Tvib : 0.0;
   J.c_rot = J.noneq ? -HC_K / r.Trot : 0.0;
   J.c_ref = HC_K / r.Tref;
@@ -1212,6 +1240,60 @@ void radis_ldm_db_destroy(radis_ldm_db* db) {
 int radis_ldm_db_size(const radis_ldm_db* db, int64_t* n_lines) {
   if (!db || !n_lines) return fail(RADIS_LDM_EINVAL, "null argument");
   *n_lines = db->n;
+  return 0;
+}
+
+int radis_ldm_db_set_qspline(radis_ldm_db* db, int iso, const double* knots, int32_t n_knots, const double* coefs,
+                             int32_t degree, double T_lo, double T_hi) {
+  if (!db || !knots || !coefs) return fail(RADIS_LDM_EINVAL, "null argument");
+  if (iso < 0 || iso >= LDM_MAX_ISO) return fail(RADIS_LDM_EINVAL, "isotopologue slot %d not in [0, %d)", iso, LDM_MAX_ISO);
+  if (degree < 1 || degree > LDM_QS_MAXDEG) return fail(RADIS_LDM_EINVAL, "spline degree %d not in [1, %d]", degree, LDM_QS_MAXDEG);
+  if (n_knots < 2 * (degree + 1) || n_knots > LDM_QS_MAXK)
+    return fail(RADIS_LDM_EINVAL, "%d knots: need %d to %d for degree %d", n_knots, 2 * (degree + 1), LDM_QS_MAXK, degree);
+  if (!(T_lo < T_hi) || !std::isfinite(T_lo) || !std::isfinite(T_hi) || !(T_lo > 0))
+    return fail(RADIS_LDM_EINVAL, "bad temperature range [%g, %g]", T_lo, T_hi);
+  LdmQSpline q{};
+  q.n = n_knots;
+  q.k = degree;
+  q.lo = T_lo;
+  q.hi = T_hi;
+  for (int i = 0; i < n_knots; ++i) {
+    if (!std::isfinite(knots[i]) || (i > 0 && knots[i] < knots[i - 1]))
+      return fail(RADIS_LDM_EINVAL, "knots must be finite and non-decreasing (knot %d)", i);
+    q.t[i] = knots[i];
+  }
+  if (!(knots[degree + 1] > knots[degree]) || !(knots[n_knots - degree - 1] > knots[n_knots - degree - 2]))
+    return fail(RADIS_LDM_EINVAL, "empty first or last knot interval");
+  for (int i = 0; i < n_knots - degree - 1; ++i) {
+    if (!std::isfinite(coefs[i])) return fail(RADIS_LDM_EINVAL, "coefficient %d is not finite", i);
+    q.c[i] = coefs[i];
+  }
+  HIPCHK(hipSetDevice(db->ctx->device));
+  if (!db->qs_mem) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(LdmQSpline) * LDM_MAX_ISO));
+    db->qs_mem = (LdmQSpline*)p;
+    db->ctx->in_use += (int64_t)(sizeof(LdmQSpline) * LDM_MAX_ISO);
+    db->h_qs.assign(LDM_MAX_ISO, LdmQSpline{});
+  }
+  HIPCHK(hipMemcpy(db->qs_mem + iso, &q, sizeof q, hipMemcpyHostToDevice));
+  db->h_qs[iso] = q;
+  db->dev.qs = db->qs_mem;
+  db->dev.qs_mask |= 1u << iso;
+  return 0;
+}
+
+int radis_ldm_qspline_eval(const radis_ldm_db* db, int iso, const double* T, int64_t n, double* Q, void* stream) {
+  if (!db) return fail(RADIS_LDM_EINVAL, "null database");
+  if (iso < 0 || iso >= LDM_MAX_ISO || !(db->dev.qs_mask & (1u << iso)))
+    return fail(RADIS_LDM_EINVAL, "isotopologue slot %d has no partition-function spline", iso);
+  if (n < 0 || (n > 0 && (!T || !Q))) return fail(RADIS_LDM_EINVAL, "bad arguments");
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(db->ctx->device));
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_qspline_eval, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const LdmQSpline*)db->qs_mem,
+                     iso, T, (long long)n, Q);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -34,6 +34,17 @@ static_assert(LDM_AX_CHUNK == LDM_SKY_BLOCK, "a partial block is one scan chunk"
 #define LDM_ST_FOLD 4      // alias folding did not converge
 #define LDM_ST_WIDTH 8     // non-positive broadening width
 
+// Tabulated partition function of one isotopologue slot as a B-spline of
+// Q(T) in scipy's splrep form: knots t[0..n), coefficients c[0..n-k-1),
+// degree k; valid (as the reference's table allows) for T in [lo, hi].
+#define LDM_QS_MAXK 256
+#define LDM_QS_MAXDEG 5
+struct LdmQSpline {
+  double t[LDM_QS_MAXK], c[LDM_QS_MAXK];
+  int n, k;
+  double lo, hi;
+};
+
 struct LdmDb {
   const double* nu0;
   const float* S_ref;
@@ -57,6 +68,8 @@ struct LdmDb {
   int sky_nb;
   int n_iso_slots;
   const int2* iso_rng;   // [sky_nb][n_iso_slots]: first/last line of each iso in the block (-1: none)
+  const LdmQSpline* qs;  // [LDM_MAX_ISO] partition-function splines (slots in qs_mask), may be null
+  unsigned qs_mask;
 };
 
 struct LdmJob {
@@ -84,6 +97,8 @@ struct LdmJob {
   int fp64_widths;               // 1: log-width positions always in FP64
   int min_rms;                   // 1: min-RMS weights
   int noneq;                     // 1: two-temperature (Tvib, Trot) line strengths
+  int device_q;                  // 1: qratio/qi_f of the database's spline slots computed by k_finalize
+  double Tref;                   // reference temperature of S_ref (device Q ratio)
   double c_vib, c_rot, c_ref;    // -hc_k/Tvib, -hc_k/Trot, hc_k/Tref
   float mr_dv, mr_dxG, mr_dxL;   // wstep, dxG, dxL of the min-RMS corrections
   float mr_c1gg, mr_c2gg, mr_c1lg, mr_c2lg;

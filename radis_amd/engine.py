@@ -126,6 +126,7 @@ class LineDatabase:
         self.molar_mass = mm
         self.Q = get("Q")
         self.Qneq = get("Qneq")
+        self.q_tables = {}  # iso slot -> (knots, coefficients, degree, T_lo, T_hi) on the device
         self.order = order
         self.nu0 = nu0
         self.n = int(nu0.size)
@@ -167,6 +168,41 @@ class LineDatabase:
         check(self._lib.radis_ldm_request_bytes(self.handle, byref(req), byref(b), byref(inf)))
         return int(b.value), inf
 
+    def set_partition_table(self, iso, T, Q, k=3):
+        """Tabulated Q(T) of isotopologue slot ``iso``, interpolated as
+        PartFuncCO2_CDSDtab does (radis/levels/partfunc_cdsd.py:84-117):
+        ``scipy.interpolate.splrep(T, Q)`` (cubic), valid from 0.95 min(T) to
+        1.05 max(T).  The spline goes to the device, which evaluates
+        Q(Tref)/Q(T) itself for requests made with ``device_q``."""
+        from scipy.interpolate import splrep
+
+        T = np.asarray(T, dtype=np.float64).ravel()
+        Q = np.asarray(Q, dtype=np.float64).ravel()
+        if T.size != Q.size or T.size < k + 1:
+            raise ValueError("T and Q must have the same length, above the spline degree")
+        t, c, kk = splrep(T, Q, k=k)
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        c = np.ascontiguousarray(np.asarray(c, dtype=np.float64)[: t.size - kk - 1])
+        lo, hi = 0.95 * float(T.min()), 1.05 * float(T.max())  # partfunc_cdsd.py:101-103
+        check(self._lib.radis_ldm_db_set_qspline(self.handle, int(iso), t.ctypes.data_as(POINTER(ctypes.c_double)),
+                                                 t.size, c.ctypes.data_as(POINTER(ctypes.c_double)), int(kk), lo, hi))
+        self.q_tables[int(iso)] = (t, c, int(kk), lo, hi)
+
+    def partition_function(self, iso, T, stream=None):
+        """Q(T) of slot ``iso`` from its table, evaluated on the device."""
+        torch = _torch()
+        Tn = np.asarray(T, dtype=np.float64)
+        Td = torch.as_tensor(np.ascontiguousarray(Tn.ravel()), device="cuda")
+        Qd = torch.empty_like(Td)
+        check(self._lib.radis_ldm_qspline_eval(self.handle, int(iso), c_void_p(Td.data_ptr()), Td.numel(),
+                                               c_void_p(Qd.data_ptr()), _stream_handle(stream)))
+        return Qd.cpu().numpy().reshape(Tn.shape)
+
+    def tables_cover_isotopologues(self):
+        """True when every isotopologue slot with a molar mass has a Q table."""
+        slots = [s for s in range(1, self.n_iso_slots) if self.molar_mass[s] > 0]
+        return bool(slots) and all(s in self.q_tables for s in slots)
+
     def q_ratio(self, T, Tref):
         """Q(Tref)/Q(T) per iso slot; slots without lines get 1."""
         out = np.ones(MAX_ISO)
@@ -202,7 +238,7 @@ class LineDatabase:
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
                  fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0,
-                 generic_fft=False, optimization="simple", Tvib=None, Trot=None):
+                 generic_fft=False, optimization="simple", Tvib=None, Trot=None, device_q=None):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
@@ -213,7 +249,10 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     (radis/lbl/broadening.py:2265-2318).  ``Tvib`` and ``Trot`` select
     two-temperature line strengths (``Tgas`` is then the translational
     temperature of the widths; RADIS's ``non_eq_spectrum`` takes
-    ``Ttrans = Trot`` by default, radis/lbl/factory.py:1685-1687)."""
+    ``Ttrans = Trot`` by default, radis/lbl/factory.py:1685-1687).
+    ``device_q`` takes Q(Tref)/Q(T) from the database's partition tables,
+    evaluated on the device (default: when no ``q_ratio`` is given and the
+    tables cover every isotopologue slot of an equilibrium request)."""
     if optimization not in ("simple", "min-RMS"):
         raise ValueError(f"optimization must be 'simple' or 'min-RMS', not {optimization!r}")
     r = _native.Request()
@@ -223,7 +262,13 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     noneq = Tvib is not None or Trot is not None
     if noneq and (Tvib is None or Trot is None):
         raise ValueError("give both Tvib and Trot")
-    if q_ratio is not None:
+    if device_q is None:  # the database's Q tables when they cover its isotopologues
+        device_q = q_ratio is None and not noneq and db.tables_cover_isotopologues()
+    if device_q:
+        if noneq:
+            raise ValueError("device_q takes equilibrium partition functions only")
+        q = np.ones(MAX_ISO)  # ignored for the slots with a table
+    elif q_ratio is not None:
         q = np.asarray(q_ratio, dtype=np.float64)
     elif noneq:
         q = db.q_ratio_noneq(Tvib, Trot, Tref)
@@ -234,7 +279,8 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     r.fft_len = int(fft_len or 0)
     r.n_v = int(n_v or 0)
     r.flags = ((_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
-               | (_native.MIN_RMS if optimization == "min-RMS" else 0) | (_native.NONEQ if noneq else 0))
+               | (_native.MIN_RMS if optimization == "min-RMS" else 0) | (_native.NONEQ if noneq else 0)
+               | (_native.DEVICE_Q if device_q else 0))
     if noneq:
         r.Tvib, r.Trot = float(Tvib), float(Trot)
     r.fft_n1, r.tile_w = int(fft_n1 or 0), int(tile_w or 0)

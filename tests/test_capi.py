@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _native.load().radis_ldm_abi_version() == _native.ABI_VERSION == 3
+    assert _native.load().radis_ldm_abi_version() == _native.ABI_VERSION == 4
 
 
 @pytest.mark.parametrize("wmin,wmax,wstep", [(1900.0, 2300.0, 0.002), (2000.0, 2500.0, 0.001),
