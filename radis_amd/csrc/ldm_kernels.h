@@ -168,35 +168,37 @@ __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const i
     const int gb0 = it.z, G = -it.w;
     const int dmax = J.log_r > 0.0 ? 2 : 0, dmin = dmax + 1;
     const int stride = db.sky_nb + 1;
-    {
-      const int* off = db.sky_off + dmax * stride;
-      for (int c = off[gb0] + (int)threadIdx.x; c < off[gb0 + G]; c += blockDim.x) {
-        const float4 q = db.sky[c];
-        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, q.x, q.y, q.z, q.w));
+    const int* offa = db.sky_off + dmax * stride;
+    const int* offb = db.sky_off + dmin * stride;
+    const int a0 = offa[gb0], na = offa[gb0 + G] - a0;
+    const int b0 = offb[gb0], nb = offb[gb0 + G] - b0;
+    // Gaussian FWHM = nu0 * const(iso): per isotopologue the block's first
+    // and last line (nu0 sorted)
+    const int ns = db.n_iso_slots;
+    const int nc = J.single_iso ? 0 : G * ns;
+    // one pass over the two candidate lists and the isotopologue ranges, so
+    // that their loads are in flight together
+    const int nmax = max(max(na, nb), nc);
+    for (int c = threadIdx.x; c < nmax; c += blockDim.x) {
+      float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa;
+      int2 r = make_int2(-1, -1);
+      if (c < na) qa = db.sky[a0 + c];
+      if (c < nb) qb = db.sky[b0 + c];
+      if (c < nc) r = db.iso_rng[(long long)(gb0 + c / ns) * ns + (c - (c / ns) * ns)];
+      if (c < na) {
+        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, qa.x, qa.y, qa.z, qa.w));
         lmax = bl > lmax ? bl : lmax;
       }
-    }
-    {
-      const int* off = db.sky_off + dmin * stride;
-      for (int c = off[gb0] + (int)threadIdx.x; c < off[gb0 + G]; c += blockDim.x) {
-        const float4 q = db.sky[c];
-        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, q.x, q.y, q.z, q.w));
+      if (c < nb) {
+        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, qb.x, qb.y, qb.z, qb.w));
         lmin = bl < lmin ? bl : lmin;
       }
-    }
-    if (!J.single_iso) {
-      // Gaussian FWHM = nu0 * const(iso): per isotopologue the block's first
-      // and last line (nu0 sorted)
-      const int ns = db.n_iso_slots;
-      for (int q = threadIdx.x; q < G * ns; q += blockDim.x) {
-        const int b = gb0 + q / ns, s = q - (q / ns) * ns;
-        const int2 r = db.iso_rng[(long long)b * ns + s];
-        if (r.x >= 0) {
-          const unsigned long long b0 = double_to_bits(fwhm_gauss(db.nu0[r.x], J.dopp[s]));
-          const unsigned long long b1 = double_to_bits(fwhm_gauss(db.nu0[r.y], J.dopp[s]));
-          gmin = b0 < gmin ? b0 : gmin;
-          gmax = b1 > gmax ? b1 : gmax;
-        }
+      if (r.x >= 0) {
+        const int s = c - (c / ns) * ns;
+        const unsigned long long g0 = double_to_bits(fwhm_gauss(db.nu0[r.x], J.dopp[s]));
+        const unsigned long long g1 = double_to_bits(fwhm_gauss(db.nu0[r.y], J.dopp[s]));
+        gmin = g0 < gmin ? g0 : gmin;
+        gmax = g1 > gmax ? g1 : gmax;
       }
     }
   } else {
@@ -346,14 +348,29 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
                                                  LdmResult* __restrict__ res, LdmDb db) {
   LdmJob& J = jobs[blockIdx.x];
   LdmAxes& A = axes[blockIdx.x];
-  __shared__ double wgax[LDM_MAX_AX], wlax[LDM_MAX_AX];
+  __shared__ double wgax[LDM_MAX_AX], wlax[LDM_MAX_AX], lgax[LDM_MAX_AX], llax[LDM_MAX_AX];
   __shared__ double s_wg_min, s_wl_min, s_wg_max, s_wl_max;
   __shared__ int s_ng, s_nl, s_status;
-  __shared__ unsigned long long s_ext[4];
-  // the status is this run's own (the job table is not re-copied when the
-  // host has nothing new for it); the reduction's barriers order this reset
-  // before the fold-count atomics below
-  if (threadIdx.x == 0) J.status = 0;
+  __shared__ unsigned long long s_w[4][4];
+  // This kernel is a chain of dependent memory round trips, so everything it
+  // reads is loaded up front: the job's values, the partial extremes and,
+  // for a single isotopologue, the window's end lines (its Gaussian width
+  // extremes).  The axis nodes and logs stay in LDS, and the status is
+  // written once at the end (it is this run's own: the job table is not
+  // re-copied when the host has nothing new for it).
+  const long long nch = J.n_ax_part;
+  const unsigned long long* part = J.ax_part;
+  const long long i0 = J.i0, i1 = J.i1;
+  const bool single = J.single_iso;
+  const double dopp1 = J.dopp[1], dxG = J.dxG, dxL = J.dxL, wstep = J.wstep, fold_thresh = J.fold_thresh;
+  const long long L = J.L;
+  const int m_cap = J.m_cap;
+  int* const nfold = J.nfold;
+  float* const row_a = J.row_a;
+  float* const row_b = J.row_b;
+  float* const row_inv_f0 = J.row_inv_f0;
+  const bool ends = threadIdx.x == 0 && single && i1 > i0;
+  const double nu_a = ends ? db.nu0[i0] : 0.0, nu_b = ends ? db.nu0[i1 - 1] : 0.0;
   // Q(Tref)/Q(T) of the slots with a partition-function spline (the fill
   // reads these after this kernel): calc_linestrength_eq's Qref/Q,
   // radis/lbl/base.py:2605-2697
@@ -369,9 +386,8 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
   {
     // reduce the per-item width extremes written by k_axis
     unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
-    const long long nch = J.n_ax_part;
     for (long long c = threadIdx.x; c < nch; c += blockDim.x) {
-      const unsigned long long* P = J.ax_part + 4 * c;
+      const unsigned long long* P = part + 4 * c;
       lmin = P[0] < lmin ? P[0] : lmin;
       lmax = P[1] > lmax ? P[1] : lmax;
       gmin = P[2] < gmin ? P[2] : gmin;
@@ -384,7 +400,6 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
       o = __shfl_xor(gmin, off); gmin = o < gmin ? o : gmin;
       o = __shfl_xor(gmax, off); gmax = o > gmax ? o : gmax;
     }
-    __shared__ unsigned long long s_w[4][4];
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) { s_w[0][w] = lmin; s_w[1][w] = lmax; s_w[2][w] = gmin; s_w[3][w] = gmax; }
     __syncthreads();
@@ -395,90 +410,88 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
         gmin = s_w[2][k] < gmin ? s_w[2][k] : gmin;
         gmax = s_w[3][k] > gmax ? s_w[3][k] : gmax;
       }
-      s_ext[0] = lmin; s_ext[1] = lmax; s_ext[2] = gmin; s_ext[3] = gmax;
+      int status = 0, ng = 0, nl = 0;
+      double wg_min = 0, wl_min = 0, wg_max = 0, wl_max = 0;
+      if (i1 > i0) {
+        if (single) {
+          wg_min = fwhm_gauss(nu_a, dopp1);
+          wg_max = fwhm_gauss(nu_b, dopp1);
+        } else {
+          wg_min = bits_to_double(gmin);
+          wg_max = bits_to_double(gmax);
+        }
+        wl_min = bits_to_double(lmin);
+        wl_max = bits_to_double(lmax);
+        if (!(wg_min > 0.0) || !(wl_min > 0.0)) {
+          status |= LDM_ST_WIDTH;
+        } else {
+          // N = ceil((log(w_max + 1e-4) - log(w_min)) / dx) + 1, radis/lbl/broadening.py:1835-1836
+          ng = (int)ceil((log(wg_max + 1e-4) - log(wg_min)) / dxG) + 1;
+          nl = (int)ceil((log(wl_max + 1e-4) - log(wl_min)) / dxL) + 1;
+          if (ng > LDM_MAX_AX || nl > LDM_MAX_AX || ng < 2 || nl < 2) status |= LDM_ST_AXIS;
+          else if (ng * nl > m_cap) status |= LDM_ST_CAP;
+        }
+      }
+      if (status) { ng = 0; nl = 0; }
+      s_ng = ng; s_nl = nl; s_status = status;
+      s_wg_min = wg_min; s_wl_min = wl_min;
+      s_wg_max = wg_max; s_wl_max = wl_max;
+      A.ng = ng;
+      A.nl = nl;
+      A.m = ng * nl;
+      A.p = (ng * nl + 1) / 2;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    int status = 0, ng = 0, nl = 0;
-    double wg_min = 0, wl_min = 0, wg_max = 0, wl_max = 0;
-    if (J.i1 > J.i0) {
-      if (J.single_iso) {
-        wg_min = fwhm_gauss(db.nu0[J.i0], J.dopp[1]);
-        wg_max = fwhm_gauss(db.nu0[J.i1 - 1], J.dopp[1]);
-      } else {
-        wg_min = bits_to_double(s_ext[2]);
-        wg_max = bits_to_double(s_ext[3]);
-      }
-      wl_min = bits_to_double(s_ext[0]);
-      wl_max = bits_to_double(s_ext[1]);
-      if (!(wg_min > 0.0) || !(wl_min > 0.0)) {
-        status |= LDM_ST_WIDTH;
-      } else {
-        // N = ceil((log(w_max + 1e-4) - log(w_min)) / dx) + 1, radis/lbl/broadening.py:1835-1836
-        ng = (int)ceil((log(wg_max + 1e-4) - log(wg_min)) / J.dxG) + 1;
-        nl = (int)ceil((log(wl_max + 1e-4) - log(wl_min)) / J.dxL) + 1;
-        if (ng > LDM_MAX_AX || nl > LDM_MAX_AX || ng < 2 || nl < 2) status |= LDM_ST_AXIS;
-        else if (ng * nl > J.m_cap) status |= LDM_ST_CAP;
-      }
-    }
-    if (status) { ng = 0; nl = 0; }
-    s_ng = ng; s_nl = nl; s_status = status;
-    s_wg_min = wg_min; s_wl_min = wl_min;
-    s_wg_max = wg_max; s_wl_max = wl_max;
-    A.ng = ng;
-    A.nl = nl;
-    A.m = ng * nl;
-    A.p = (ng * nl + 1) / 2;
-  }
-  __syncthreads();
   // axis nodes w_min * exp(dx * i) and their logs, radis/lbl/broadening.py:1837
   for (int i = threadIdx.x; i < s_ng; i += blockDim.x) {
-    wgax[i] = s_wg_min * exp(J.dxG * (double)i);
-    A.laG[i] = log(wgax[i]);
+    wgax[i] = s_wg_min * exp(dxG * (double)i);
+    lgax[i] = log(wgax[i]);
+    A.laG[i] = lgax[i];
   }
   for (int i = threadIdx.x; i < s_nl; i += blockDim.x) {
-    wlax[i] = s_wl_min * exp(J.dxL * (double)i);
-    A.laL[i] = log(wlax[i]);
+    wlax[i] = s_wl_min * exp(dxL * (double)i);
+    llax[i] = log(wlax[i]);
+    A.laL[i] = llax[i];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i + 1 < s_ng; i += blockDim.x) A.slG[i] = 1.0 / (A.laG[i + 1] - A.laG[i]);
-  for (int i = threadIdx.x; i + 1 < s_nl; i += blockDim.x) A.slL[i] = 1.0 / (A.laL[i + 1] - A.laL[i]);
+  for (int i = threadIdx.x; i + 1 < s_ng; i += blockDim.x) A.slG[i] = 1.0 / (lgax[i + 1] - lgax[i]);
+  for (int i = threadIdx.x; i + 1 < s_nl; i += blockDim.x) A.slL[i] = 1.0 / (llax[i + 1] - llax[i]);
   if (threadIdx.x == 0 && s_ng > 0) {
     // FP32 fast positions work in log2 units (the hardware logarithm)
     const double LOG2E = 1.4426950408889634, LN2 = 0.6931471805599453;
-    J.laG0_f = (float)(A.laG[0] * LOG2E);
-    J.laL0_f = (float)(A.laL[0] * LOG2E);
-    J.inv_dxG_f = (float)(LN2 / J.dxG);
-    J.inv_dxL_f = (float)(LN2 / J.dxL);
+    J.laG0_f = (float)(lgax[0] * LOG2E);
+    J.laL0_f = (float)(llax[0] * LOG2E);
+    J.inv_dxG_f = (float)(LN2 / dxG);
+    J.inv_dxL_f = (float)(LN2 / dxL);
   }
-  __syncthreads();
   const int ng = s_ng, nl = s_nl;
   int status = 0;
   const double PI = 3.141592653589793;
-  const double h = 0.5 / J.wstep;                      // n/(2 wstep) step
-  const double xrev0 = (double)(J.L / 2) * (1.0 / ((double)J.L * J.wstep));  // rfftfreq(L, wstep)[-1]
+  const double h = 0.5 / wstep;                      // n/(2 wstep) step
+  const double xrev0 = (double)(L / 2) * (1.0 / ((double)L * wstep));  // rfftfreq(L, wstep)[-1]
   for (int r = threadIdx.x; r < ng * nl; r += blockDim.x) {
     const int l = r / nl, m = r - (r / nl) * nl;
     const double wG = wgax[l], wL = wlax[m];
     int n = 1;
-    while (n < LDM_NFOLD_MAX && voigt_ft64(n * h, wG, wL) >= J.fold_thresh) ++n;
+    while (n < LDM_NFOLD_MAX && voigt_ft64(n * h, wG, wL) >= fold_thresh) ++n;
     if (n >= LDM_NFOLD_MAX) status |= LDM_ST_FOLD;
     const int nf = n - 1;
     double f0 = 1.0;
     for (int q = 1; q <= nf; ++q) f0 += voigt_ft64(q * h + ((q & 1) ? xrev0 : 0.0), wG, wL);
-    J.nfold[r] = nf;
-    J.row_a[r] = (float)((PI * wG) * (PI * wG) / (4.0 * 0.6931471805599453));
-    J.row_b[r] = (float)(PI * wL);
-    J.row_inv_f0[r] = (float)(1.0 / f0);
+    nfold[r] = nf;
+    row_a[r] = (float)((PI * wG) * (PI * wG) / (4.0 * 0.6931471805599453));
+    row_b[r] = (float)(PI * wL);
+    row_inv_f0[r] = (float)(1.0 / f0);
   }
-  if (status) atomicOr(&J.status, status);
+  if (status) atomicOr(&s_status, status);
   __syncthreads();
   if (threadIdx.x == 0) {
-    J.status |= s_status;
-    res[blockIdx.x].ng = A.ng;
-    res[blockIdx.x].nl = A.nl;
-    res[blockIdx.x].status = J.status;
+    const int st = s_status;
+    J.status = st;
+    res[blockIdx.x].ng = ng;
+    res[blockIdx.x].nl = nl;
+    res[blockIdx.x].status = st;
     res[blockIdx.x].wg_min = s_wg_min;
     res[blockIdx.x].wg_max = s_wg_max;
     res[blockIdx.x].wl_min = s_wl_min;
