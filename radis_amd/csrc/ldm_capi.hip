@@ -505,6 +505,16 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.wmin = r.wmin;
   J.delta_w = R.delta_w;
   J.inv_delta_w = 1.0 / R.delta_w;
+  {
+    // w[j+1] - w[j] is delta_w up to the rounding of the two grid values
+    // (a few ulp of the largest |w|); below one step minus that and 1e-7 of
+    // a step, x - w[j] >= 0 puts x inside [w[j], w[j+1]) with NumPy's
+    // position rounding to j (grid_index)
+    const double wlast = r.wmin + (double)(R.nv - 1) * R.delta_w;
+    const double wabs = std::max(std::fabs(r.wmin), std::fabs(wlast));
+    J.grid_fast_hi = R.delta_w * (1.0 - 1e-7) - 8.0 * 2.220446049250313e-16 * wabs;
+    J.inv_delta_f = (float)(1.0 / R.delta_w);
+  }
   J.wstep = r.wstep;
   J.nv = R.nv;
   J.v = R.nv + 1;

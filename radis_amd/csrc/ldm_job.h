@@ -75,6 +75,8 @@ struct LdmDb {
 struct LdmJob {
   double wmin, delta_w, wstep;   // grid w[i] = wmin + i*delta_w
   double inv_delta_w;            // 1/delta_w (first guess of a grid index only)
+  double grid_fast_hi;           // x - w[j] below this: j is x's interval and NumPy cannot round to j+1
+  float inv_delta_f;             // 1/delta_w in FP32 (fraction of the common case)
   long long nv, v;               // N_v and V = N_v + 1 (written LDM length)
   long long L;                   // FFT length = n1*n2
   long long i0, i1;              // window line range

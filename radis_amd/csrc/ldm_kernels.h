@@ -66,8 +66,12 @@ __device__ __forceinline__ double grid_position(double x, double wmin, double de
 // below 2e-9 for j < 2^24), so away from that the index is j and the
 // fraction, which only feeds FP32 weights, takes an FP32 reciprocal instead
 // of the FP64 division.
-__device__ __forceinline__ void grid_index(double x, double wmin, double delta, double inv_delta, int nv, int& k0,
-                                           float& tv) {
+// Common case: 0 <= x - w[j] < fast_hi (one step less a margin above the
+// rounding of w[j+1] and 1e-7 of a step), so the first guess j is x's
+// interval and the position rounds to j; the fraction then takes the FP32
+// reciprocal of the step (w[j+1] - w[j] differs from it by rounding only).
+__device__ __forceinline__ void grid_index(double x, double wmin, double delta, double inv_delta, int nv,
+                                           double fast_hi, float inv_delta_f, int& k0, float& tv) {
 #pragma clang fp contract(off)
   const double wlast = wmin + (double)(nv - 1) * delta;
   if (x < wmin) { k0 = 0; tv = 0.0f; return; }
@@ -75,6 +79,12 @@ __device__ __forceinline__ void grid_index(double x, double wmin, double delta, 
   int j = (int)((x - wmin) * inv_delta);
   j = j > nv - 2 ? nv - 2 : j;
   double wj = wmin + (double)j * delta;
+  const double d0 = x - wj;
+  if (d0 >= 0.0 && d0 < fast_hi) {
+    k0 = j;
+    tv = (float)d0 * inv_delta_f;
+    return;
+  }
   if (wj > x && j > 0) {
     --j;
     wj = wmin + (double)j * delta;
@@ -674,7 +684,8 @@ __device__ __forceinline__ void iso_vals(const LdmJob& J, int iso, IsoVals& v) {
 // Request constants of the per-line computation, read once per workgroup;
 // the per-isotopologue values live in LDS (indexed per lane).
 struct LineConst {
-  double wmin, delta_w, inv_delta_w;
+  double wmin, delta_w, inv_delta_w, grid_fast_hi;
+  float inv_delta_f;
   double p_atm, x_air, x_self, log_r;
   double c_boltz, i_scale;
   float c_stim_T, c_stim_R, log_r_f, pa_air_f, pa_self_f;
@@ -683,7 +694,8 @@ struct LineConst {
   double c_vib, c_rot, c_ref;
   const LdmJob* job;
   __device__ explicit LineConst(const LdmJob& J, const LdmDb& db)
-      : wmin(J.wmin), delta_w(J.delta_w), inv_delta_w(J.inv_delta_w), p_atm(J.p_atm), x_air(J.x_air),
+      : wmin(J.wmin), delta_w(J.delta_w), inv_delta_w(J.inv_delta_w), grid_fast_hi(J.grid_fast_hi),
+        inv_delta_f(J.inv_delta_f), p_atm(J.p_atm), x_air(J.x_air),
         x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
         c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
         laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
@@ -755,7 +767,7 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 #pragma clang fp contract(off)
     nu = nu0 + ((double)r.dl * C.p_atm);
   }
-  grid_index(nu, C.wmin, C.delta_w, C.inv_delta_w, C.nv, t.k0, t.tv);
+  grid_index(nu, C.wmin, C.delta_w, C.inv_delta_w, C.nv, C.grid_fast_hi, C.inv_delta_f, t.k0, t.tv);
   const int iso = r.iso;
   const float nuf = (float)nu0;
   // FP32 positions on the log axes
