@@ -236,10 +236,15 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
   constexpr int NSL = fc_ns(N1, fc_nstages(N1) - 1);
   __shared__ float2 buf[LdsSize<CAPE>::value];
   __shared__ float2 s_tw[N1];
-  // Persistent: workgroup g takes the column blocks g, g + G, g + 2G, ...
-  // (G = gridDim.x <= n_items).  A block's first-stage inputs sit in v; the
-  // next block's loads go into v right after the first stage has consumed
-  // them, so they are in flight during the LDS stages and the stores.
+  // Persistent: workgroup g takes the column blocks g', g' + G, g' + 2G, ...
+  // (G = gridDim.x <= n_items).  Blocks narrower than a 128-B line take
+  // g' = xcd_block(g, G): the workgroups of one XCD then take consecutive
+  // blocks, so the two halves of a line meet in one L2 (N1 = 640, 100 M
+  // lines: 339 -> 330 us); full-line blocks keep g' = g (the remap cost
+  // them 2.6 us at N1 = 512).  A block's first-stage inputs
+  // sit in v; the next block's loads go into v right after the first stage
+  // has consumed them, so they are in flight during the LDS stages and the
+  // stores.
   float2 v[SH::IT0][SH::R0];
   // issue the loads of block idx into v; false (for the whole workgroup)
   // when the block has no work
@@ -270,9 +275,10 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
     return true;
   };
   // W_N1 table: the same for every request of this N1
-  fc_load_tw<N1, T>(s_tw, jobs[items[blockIdx.x].x].tw1);
-  bool live = fetch(blockIdx.x);
-  for (int idx = blockIdx.x; idx < n_items; idx += gridDim.x) {
+  const int first = B < 32 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  fc_load_tw<N1, T>(s_tw, jobs[items[first].x].tw1);
+  bool live = fetch(first);
+  for (int idx = first; idx < n_items; idx += gridDim.x) {
     if (!live) {
       live = fetch(idx + gridDim.x);
       continue;
