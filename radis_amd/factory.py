@@ -16,7 +16,7 @@ import time
 
 import numpy as np
 
-from . import engine
+from . import engine, partfunc
 from .engine import DXG_DEFAULT, DXL_DEFAULT, FOLDING_THRESH_DEFAULT
 
 class Spectrum:
@@ -140,13 +140,22 @@ class SpectrumFactory:
         self.optimization = optimization
         self.verbose = verbose
         self.lines = None
+        self.parfunc = None
         if lines is not None:
-            self.load_lines(lines)
+            self.load_lines(lines, parfunc=kwargs.get("parfunc"), parfuncfmt=kwargs.get("parfuncfmt", "cdsd"))
 
-    def load_lines(self, lines):
+    def load_lines(self, lines, parfunc=None, parfuncfmt="cdsd"):
         """Use ``lines`` (a LineList or a mapping of columns) as the line
-        database; replaces load_databank/fetch_databank."""
+        database; replaces load_databank/fetch_databank.  ``parfunc`` is a
+        tabulated partition-function file (``parfuncfmt='cdsd'``: the CDSD
+        CO2 table format, radis/lbl/loader.py:162, 1023-1060, 1188;
+        radis/levels/partfunc_cdsd.py:44-117), interpolated and evaluated
+        on the device for every isotopologue of the lines."""
+        if parfunc is not None and parfuncfmt != "cdsd":
+            raise NotImplementedError(f"parfuncfmt={parfuncfmt!r} (this package reads 'cdsd' tables; "
+                                      "'hapi' needs the HITRAN Python interface)")
         self.lines = lines
+        self.parfunc = parfunc
         return self
 
     @property
@@ -212,6 +221,9 @@ class _LineApp:
     def __init__(self, sf, cols, Nv, device_id):
         self._dev = engine.Device(int(device_id))
         self._db = engine.LineDatabase(self._dev, cols)
+        if sf.parfunc is not None:
+            isos = [s for s in range(1, self._db.n_iso_slots) if self._db.molar_mass[s] > 0]
+            partfunc.set_cdsd_co2_tables(self._db, sf.parfunc, isotopes=isos)
         self.sf = sf
         self.Nv = Nv
         self._batch = None

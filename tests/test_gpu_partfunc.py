@@ -97,3 +97,26 @@ def test_device_q_errors(dev):
     with pytest.raises(_native.LdmError):
         db.partition_function(7, [300.0])
     db.close()
+
+
+def test_factory_parfunc_file():
+    """SpectrumFactory with a tabulated partition-function file (the
+    load_databank ``parfunc`` argument, radis/lbl/loader.py:1023-1060, 1188):
+    Q comes from the table, evaluated on the device."""
+    from radis_amd.factory import SpectrumFactory
+
+    lines = _lines(20_000, seed=11)
+    cols = {k: v for k, v in lines.items() if k != "Q"}
+    sf = SpectrumFactory(2200.0, 2300.0, wstep=0.002, pressure=1.0, mole_fraction=0.1)
+    sf.load_lines(cols, parfunc=FIX, parfuncfmt="cdsd")
+    s = sf.eq_spectrum_gpu(Tgas=1800.0)
+    _, k = s.get("abscoeff")
+    ref = oracle.eq_abscoeff(lines, 2200.0, 2300.0, 0.002, 1800.0, 1.0, 0.1,
+                             L=int(s.gpu_app._batch.info(0).fft_len))
+    assert oracle.rel_peak_error(k, ref) <= 1e-4
+    k2 = s.recalc_gpu("abscoeff", Tgas=900.0)
+    ref2 = oracle.eq_abscoeff(lines, 2200.0, 2300.0, 0.002, 900.0, 1.0, 0.1, L=int(s.gpu_app._batch.info(0).fft_len))
+    assert oracle.rel_peak_error(k2, ref2) <= 1e-4
+    s.exit_gpu()
+    with pytest.raises(NotImplementedError):
+        SpectrumFactory(2200.0, 2300.0).load_lines(cols, parfunc=FIX, parfuncfmt="hapi")
