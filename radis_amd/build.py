@@ -12,9 +12,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "ldm_capi.hip")
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("ldm_capi.hip", "ldm_job.h", "ldm_kernels.h", "ldm_fft.h")] + [
-    os.path.join(ROOT, "include", "radis_ldm.h")
-]
 OUT = os.path.join(HERE, "_lib", "libradis_ldm.so")
 ARCH = os.environ.get("RADIS_AMD_ARCH", "gfx950")
 
