@@ -258,12 +258,21 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
     const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
     const bool has_b = (2 * p + 1) < axes[it.x].m;
     const float* rb = has_b ? ra + J.ldm_stride : ra;
+    // L >= 2 N_v, so V = N_v + 1 <= L/2 unless L = 2 N_v exactly: then the
+    // upper half of every column (n1 >= N1/2, n >= L/2) is zero and its
+    // loads are skipped
+    const bool upper_zero = (long long)V <= J.L / 2;
     // the two LDM rows, zero beyond V (and row b zero when the pair is odd)
 #pragma unroll
     for (int s = 0; s < SH::IT0; ++s) {
       const int it0 = threadIdx.x + s * T;
 #pragma unroll
       for (int q = 0; q < SH::R0; ++q) {
+        // s*(T/B) + q*NR0 is the smallest row n1 = e >> LB of this (s, q)
+        if (upper_zero && s * (T >> LB) + q * NR0 >= N1 / 2) {
+          v[s][q] = make_float2(0.0f, 0.0f);
+          continue;
+        }
         const int e = it0 + q * NR0 * B;
         const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
         const bool in = (SH::ITEMS0 % T == 0 || it0 < SH::ITEMS0) && n < V;

@@ -72,3 +72,36 @@ def test_compiled_splits_match_generic_and_oracle(dev, n1):
         assert e_ref <= 1e-4, (n1, n2, e_ref)
         b.close()
         db.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n1", [512, 640])
+def test_grid_filling_half_the_transform(dev, n1):
+    """N_v = L/2 exactly: the LDM's length V = N_v + 1 reaches into the
+    upper half of pass A's columns, so pass A takes its full-load path
+    (with V <= L/2 it skips that half).  Lines crowd the grid's end; the
+    point n = N_v itself only ever gets zero weight (a line on the last
+    grid point has position N_v - 1 exactly)."""
+    L = n1 * 2048
+    nv = L // 2
+    wmax = WMIN + (nv - 1) * WSTEP
+    while np.arange(WMIN, wmax + WSTEP, WSTEP).size > nv:
+        wmax -= WSTEP / 4
+    while np.arange(WMIN, wmax + WSTEP, WSTEP).size < nv:
+        wmax += WSTEP / 4
+    assert np.arange(WMIN, wmax + WSTEP, WSTEP).size == nv
+    lines = _lines(2000, wmax - 2.0, wmax + 0.5 * WSTEP, seed=n1)
+    db = engine.LineDatabase(dev, lines)
+    cond = dict(wmin=WMIN, wmax=wmax, wstep=WSTEP, Tgas=1200.0, pressure_bar=1.0, mole_fraction=0.2,
+                dxG=0.2, dxL=0.2)
+    b = engine.SpectrumBatch(db, [engine.make_request(db, **cond, fft_len=L, fft_n1=n1)])
+    out = b.results(b.run())[0]
+    inf = b.info(0)
+    assert inf.n_v == nv and inf.fft_len == L and inf.compiled_fft == 1
+    t = b.line_terms(0)
+    assert np.max(t["k0"]) == nv - 2  # lines in the last interval (the window ends at wmax)
+    ref = oracle.eq_abscoeff(lines.as_dict(), WMIN, wmax, WSTEP, cond["Tgas"], cond["pressure_bar"],
+                             cond["mole_fraction"], 0.2, 0.2, L=L)
+    assert oracle.rel_peak_error(out, ref) <= 1e-4
+    b.close()
+    db.close()
