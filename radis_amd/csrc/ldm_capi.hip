@@ -33,7 +33,7 @@ const double HC_K = K_H * K_C / K_KB * 100;  // constants.py:69
 const double ATM_PER_BAR = 1.01325;          // lbl/broadening.py:1063
 
 const int STAT_BLOCK = 4096;             // lines per host-side statistics block
-const int FILL_LDS_TARGET = 24 * 1024;   // preferred LDS bytes of one fill tile (FP64 cells; ~4 tiles per CU)
+const int FILL_LDS_TARGET = 44 * 1024;   // preferred LDS bytes of one fill tile (FP64 cells; 3 tiles per CU)
 const int FILL_LDS_MAX = 60 * 1024;      // hard limit (plus ~2.2 KB of static axis tables, < 64 KB)
 const int MIN_TILE_W = 16, MAX_TILE_W = 4096;
 const int N1_MAX = 1024, N2_MAX = 8192;
