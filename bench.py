@@ -450,8 +450,8 @@ def main():
         copy_reference(args.copy_ref)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(every box here has one MI355X)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the synthesis runs on one "
+              f"MI355X (sharding across GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch_256":
         out = bench_batch(args)
