@@ -34,8 +34,24 @@ _GQ_U, _GQ_L = (67, 82), (82, 97)
 _GU, _GL = (146, 153), (153, 160)
 
 
+def _open_text(path):
+    """A .par file, or one compressed with bzip2 or gzip (HITEMP distributes
+    its .par files as .bz2)."""
+    p = str(path)
+    if p.endswith(".bz2"):
+        import bz2
+
+        return bz2.open(p, "rt")
+    if p.endswith(".gz"):
+        import gzip
+
+        return gzip.open(p, "rt")
+    return open(p)
+
+
 def read_par(path):
-    """Columns of a .par file as a dict of numpy arrays (file order)."""
+    """Columns of a .par file (plain, .bz2 or .gz) as a dict of numpy arrays
+    (file order)."""
     cols = {k: [] for k, *_ in _FIELDS}
     extra = {"gu": [], "gl": [], "gq_u": [], "gq_l": []}
 
@@ -44,7 +60,7 @@ def read_par(path):
         txt = rec[a:b] if len(rec) >= b else ""
         return conv(txt) if txt.strip() else missing
 
-    with open(path) as f:
+    with _open_text(path) as f:
         for line in f:
             rec = line.rstrip("\n")
             if len(rec) < 67:

@@ -377,3 +377,21 @@ def test_noneq_inversion_gives_negative_strength():
         nu, nl = o.populations_noneq(1.0, g[i_l], Ev[i_l], Er[i_l], 1.0, g[i_u], Ev[i_u], Er[i_u], Tvib, Trot, Q)
         S = o.linestrength_noneq(1.0, g[i_l], g[i_u], nu, nl)
         assert np.sign(S) == sign and np.sign(nl / g[i_l] - nu / g[i_u]) == sign
+
+
+@pytest.mark.parametrize("suffix", [".bz2", ".gz"])
+def test_compressed_par_reads_like_plain(tmp_path, suffix):
+    """HITEMP ships its .par files compressed: the reader takes .bz2 and .gz."""
+    import bz2
+    import gzip
+
+    from radis_amd import hitran
+
+    fix = "tests/golden/hitran_co_3iso_2000_2300cm.par"
+    raw = open(fix, "rb").read()
+    p = tmp_path / ("lines.par" + suffix)
+    p.write_bytes(bz2.compress(raw) if suffix == ".bz2" else gzip.compress(raw))
+    a, b = hitran.read_par(fix), hitran.read_par(str(p))
+    assert a.keys() == b.keys()
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
