@@ -645,21 +645,28 @@ struct LineCols {
   __device__ __forceinline__ static T at(const T* p, unsigned byte_off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + byte_off);
   }
-  // NONEQ: also the vibrational energies (the request checked they exist)
-  template <bool NONEQ = false>
+  // NONEQ: also the vibrational energies (the request checked they exist).
+  // PLAIN: the database has no iso, delta or n_self column (iso 1, no
+  // shift, n_self = n_air), so those are not loaded.
+  template <bool NONEQ = false, bool PLAIN = false>
   __device__ __forceinline__ void load(unsigned i, LineRaw& r) const {
     const unsigned o4 = i * 4u;
     if (NONEQ) {
       r.evl = at(evl, o4);
       r.evu = at(evu, o4);
     }
-    r.iso = at(iso, i & imask);  // first: the per-isotopologue lookups wait on it alone
+    if (PLAIN) {
+      r.iso = 1;
+      r.dl = 0.0f;
+    } else {
+      r.iso = at(iso, i & imask);  // first: the per-isotopologue lookups wait on it alone
+      r.dl = at(dl, o4 & (dmask << 2));
+    }
     r.nu0 = at(nu0, i * 8u);
-    r.dl = at(dl, o4 & (dmask << 2));
     r.ga = at(ga, o4);
     r.gs = at(gs, o4);
     r.na = at(na, o4);
-    r.ns = at(ns, o4);
+    r.ns = PLAIN ? r.na : at(ns, o4);
     r.S = at(S, o4);
     r.El = at(El, o4);
   }
@@ -849,7 +856,7 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 // sits on the common path (only atomics), so atomics retire in the background.
 // ---------------------------------------------------------------------------
 
-template <bool NONEQ>
+template <bool NONEQ, bool PLAIN>
 __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, const int4 it, const LdmDb& db,
                                           double* acc, LineTables& tab) {
   const int M = A.m;
@@ -872,11 +879,13 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   LineRaw n1, n2;
   IsoVals v1;
   const unsigned first = lo + threadIdx.x;
+  // a plain database has one isotopologue: its values once, for every line
+  if (PLAIN) iso_vals(J, 1, v1);
   if (first < hi) {
-    cols.load<NONEQ>(first, n1);
-    iso_vals(J, n1.iso, v1);
+    cols.load<NONEQ, PLAIN>(first, n1);
+    if (!PLAIN) iso_vals(J, n1.iso, v1);
     const unsigned l1 = first + LDM_FILL_T;
-    cols.load<NONEQ>(l1 < hlast ? l1 : hlast, n2);
+    cols.load<NONEQ, PLAIN>(l1 < hlast ? l1 : hlast, n2);
   }
   {
     const int n2 = (M * Wp) >> 1;  // Wp is even: whole 16-byte pairs
@@ -891,9 +900,9 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     const LineRaw cur = n1;
     const IsoVals cv = v1;
     n1 = n2;
-    iso_vals(J, n1.iso, v1);
+    if (!PLAIN) iso_vals(J, n1.iso, v1);
     const unsigned i2 = i + 2 * LDM_FILL_T;
-    cols.load<NONEQ>(i2 < hlast ? i2 : hlast, n2);
+    cols.load<NONEQ, PLAIN>(i2 < hlast ? i2 : hlast, n2);
     LineTerms t;
     line_terms<NONEQ>(C, cur, cv, tab, ng, nl, t);
     const int c0 = t.k0 - (int)k_lo;  // column of corner k0 in the tile, in [-1, W-1] if touching
@@ -950,8 +959,14 @@ __global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ 
   const LdmJob& J = jobs[it.x];
   const LdmAxes& A = axes[it.x];
   if (A.m == 0 || J.status) return;
-  if (J.noneq) fill_tile<true>(J, A, it, db, acc, tab);
-  else fill_tile<false>(J, A, it, db, acc, tab);
+  const bool plain = !db.iso && !db.delta && !db.n_self;
+  if (J.noneq) {
+    if (plain) fill_tile<true, true>(J, A, it, db, acc, tab);
+    else fill_tile<true, false>(J, A, it, db, acc, tab);
+  } else {
+    if (plain) fill_tile<false, true>(J, A, it, db, acc, tab);
+    else fill_tile<false, false>(J, A, it, db, acc, tab);
+  }
 }
 
 // four-step twiddle W_L^e (forward sign)
