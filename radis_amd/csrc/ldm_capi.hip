@@ -111,6 +111,8 @@ struct radis_ldm_db {
   std::vector<float> dv_min, dv_max;  // Evib_u - Evib_l (two-temperature bound)
   bool has_sky = false;                // Lorentzian-width candidates per block (k_axis)
   std::vector<int> h_sky_off;          // host copy of the candidate offsets
+  // the whole list's Pareto sets for the 4 directions (the host's bound on N_L)
+  std::vector<float4> h_gsky[4];
   // partition-function splines (radis_ldm_db_set_qspline): device slots and host copies
   LdmQSpline* qs_mem = nullptr;
   std::vector<LdmQSpline> h_qs;
@@ -434,8 +436,23 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   auto rlo = [&](double a, double b) { return std::min(std::exp(a * lr), std::exp(b * lr)); };
   auto rhi = [&](double a, double b) { return std::max(std::exp(a * lr), std::exp(b * lr)); };
   const double x = r.mole_fraction;
-  const double wl_lo = 2 * p_atm * (rlo(na0, na1) * ga0 * (1 - x) + rlo(ns0, ns1) * gs0 * x) * (1 - 1e-9);
-  const double wl_hi = 2 * p_atm * (rhi(na0, na1) * ga1 * (1 - x) + rhi(ns0, ns1) * gs1 * x) * (1 + 1e-9);
+  double wl_lo = 2 * p_atm * (rlo(na0, na1) * ga0 * (1 - x) + rlo(ns0, ns1) * gs0 * x) * (1 - 1e-9);
+  double wl_hi = 2 * p_atm * (rhi(na0, na1) * ga1 * (1 - x) + rhi(ns0, ns1) * gs1 * x) * (1 + 1e-9);
+  if (!db.h_gsky[0].empty()) {
+    // tighter: the width's extremes over the whole list's Pareto sets (the
+    // same directions k_axis takes; a superset of the window, so still
+    // bounds); the per-block statistics above pair extremes of g and n
+    // that no single line has
+    auto wl = [&](const float4& q) {
+      return 2 * p_atm * (std::exp((double)q.z * lr) * q.x * (1 - x) + std::exp((double)q.w * lr) * q.y * x);
+    };
+    const int dmx = lr > 0.0 ? 2 : 0, dmn = dmx + 1;
+    double hi = 0, lo = 1e300;
+    for (const float4& q : db.h_gsky[dmx]) hi = std::max(hi, wl(q));
+    for (const float4& q : db.h_gsky[dmn]) lo = std::min(lo, wl(q));
+    wl_hi = std::min(wl_hi, hi * (1 + 1e-9));
+    wl_lo = std::max(wl_lo, lo * (1 - 1e-9));
+  }
   if (!(wl_lo > 0)) return fail(RADIS_LDM_EINVAL, "zero Lorentzian width in the window (check g_air/g_self and mole fraction)");
   double dmin = 1e300, dmax = 0;
   for (int s = 0; s < db.n_iso_slots; ++s) {
@@ -447,8 +464,12 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   const double wg_lo = 2 * (db.h_nu0[R.i0] / 2.99792458e10) * dmin * (1 - 1e-9);
   const double wg_hi = 2 * (db.h_nu0[R.i1 - 1] / 2.99792458e10) * dmax * (1 + 1e-9);
   if (!(wg_lo > 0)) return fail(RADIS_LDM_EINVAL, "non-positive Gaussian width (nu0 <= 0 in window)");
-  const double ngf = std::ceil((std::log(wg_hi + 1e-4) - std::log(wg_lo)) / r.dxG) + 2;
-  const double nlf = std::ceil((std::log(wl_hi + 1e-4) - std::log(wl_lo)) / r.dxL) + 2;
+  // k_axis takes N = ceil((log(w_max + 1e-4) - log(w_min)) / dx) + 1 from the
+  // exact extremes; the bounds above enclose them with a 1e-9 relative
+  // margin, far above the rounding of either side's logarithm, so the same
+  // expression on them is at least the device's N
+  const double ngf = std::ceil((std::log(wg_hi + 1e-4) - std::log(wg_lo)) / r.dxG) + 1;
+  const double nlf = std::ceil((std::log(wl_hi + 1e-4) - std::log(wl_lo)) / r.dxL) + 1;
   if (ngf > LDM_MAX_AX || nlf > LDM_MAX_AX)
     return fail(RADIS_LDM_ERANGE, "LDM axes too long (N_G<=%g, N_L<=%g; limit %d): increase dxG/dxL", ngf, nlf, LDM_MAX_AX);
   R.m_need = (int)(ngf * nlf);
@@ -1069,6 +1090,26 @@ void block_skyline(const radis_ldm_lines* L, int64_t i0, int cnt, int d, std::ve
   }
 }
 
+// The Pareto set of points (same directions and dominance as block_skyline).
+void point_skyline(std::vector<float4> pts, int d, std::vector<float4>& out) {
+  const float sg = (d == 0 || d == 2) ? 1.0f : -1.0f;
+  const float sn = (d == 1 || d == 2) ? 1.0f : -1.0f;
+  auto key = [&](const float4& q) {
+    return (double)(sg * q.x) + (double)(sg * q.y) + (double)(sn * q.z) + (double)(sn * q.w);
+  };
+  std::sort(pts.begin(), pts.end(), [&](const float4& a, const float4& b) { return key(a) > key(b); });
+  out.clear();
+  for (const float4& p : pts) {
+    bool dom = false;
+    for (const float4& q : out)
+      if (sg * q.x >= sg * p.x && sg * q.y >= sg * p.y && sn * q.z >= sn * p.z && sn * q.w >= sn * p.w) {
+        dom = true;
+        break;
+      }
+    if (!dom) out.push_back(p);
+  }
+}
+
 }  // namespace
 
 int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_db** out) {
@@ -1176,6 +1217,18 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
         sky_off[(size_t)d * (nfull + 1) + b + 1] = (int)sky_vals.size();
       }
     }
+    // the whole list's Pareto sets: every globally Pareto-maximal line is
+    // maximal in its block, so the blocks' sets plus the tail's hold them
+    std::vector<std::thread> tg;
+    for (int d = 0; d < 4; ++d)
+      tg.emplace_back([&, d] {
+        std::vector<float4> pts(sky_vals.begin() + sky_off[(size_t)d * (nfull + 1)],
+                                sky_vals.begin() + sky_off[(size_t)d * (nfull + 1) + nfull]);
+        if ((int64_t)nfull * LDM_SKY_BLOCK < n)
+          block_skyline(L, (int64_t)nfull * LDM_SKY_BLOCK, (int)(n - (int64_t)nfull * LDM_SKY_BLOCK), d, pts);
+        point_skyline(std::move(pts), d, db->h_gsky[d]);
+      });
+    for (auto& x : tg) x.join();
     if (db->has_iso) {
       iso_rng.assign((size_t)nfull * L->n_iso_slots, make_int2(-1, -1));
       for (int64_t i = 0; i < (int64_t)nfull * LDM_SKY_BLOCK; ++i) {
