@@ -700,14 +700,15 @@ struct LineConst {
   int nv, fp64_widths, has_delta, min_rms, i_fast;
   double c_vib, c_rot, c_ref;
   const LdmJob* job;
-  __device__ explicit LineConst(const LdmJob& J, const LdmDb& db)
+  const LdmAxes* axes;  // the axis tables of the rare exact path (global memory)
+  __device__ explicit LineConst(const LdmJob& J, const LdmAxes& A, const LdmDb& db)
       : wmin(J.wmin), delta_w(J.delta_w), inv_delta_w(J.inv_delta_w), grid_fast_hi(J.grid_fast_hi),
         inv_delta_f(J.inv_delta_f), p_atm(J.p_atm), x_air(J.x_air),
         x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
         c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
         laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
         fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), i_fast(J.i_fast), c_vib(J.c_vib),
-        c_rot(J.c_rot), c_ref(J.c_ref), job(&J) {}
+        c_rot(J.c_rot), c_ref(J.c_ref), job(&J), axes(&A) {}
 };
 
 // min-RMS weights, radis/lbl/broadening.py:2265-2303: the Gaussian and
@@ -732,22 +733,6 @@ __device__ __forceinline__ void min_rms_weights(const LdmJob& J, float log_wg, f
   t.tl = al;
 }
 
-// Per-workgroup LDS tables of the line computation.
-struct LineTables {
-  double laG[LDM_MAX_AX], slG[LDM_MAX_AX], laL[LDM_MAX_AX], slL[LDM_MAX_AX];
-  double dopp[LDM_MAX_ISO];
-};
-
-__device__ __forceinline__ void load_tables(const LdmJob& J, const LdmAxes& A, LineTables& T) {
-  for (int e = threadIdx.x; e < LDM_MAX_AX; e += blockDim.x) {
-    T.laG[e] = A.laG[e]; T.slG[e] = A.slG[e];
-    T.laL[e] = A.laL[e]; T.slL[e] = A.slL[e];
-  }
-  for (int e = threadIdx.x; e < LDM_MAX_ISO; e += blockDim.x) {
-    T.dopp[e] = J.dopp[e];
-  }
-}
-
 __device__ __forceinline__ bool near_node(float p, int n) {
   const float f = p - floorf(p);
   return f < LDM_TIE_EPS || f > 1.0f - LDM_TIE_EPS || p < LDM_TIE_EPS || p > (float)(n - 1) - LDM_TIE_EPS;
@@ -767,7 +752,7 @@ __device__ __forceinline__ double fwhm_lorentz_c(const LineConst& C, double g_ai
 
 template <bool NONEQ = false>
 __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r, const IsoVals& iv,
-                                           const LineTables& T, int ng, int nl, LineTerms& t) {
+                                           int ng, int nl, LineTerms& t) {
   const double nu0 = r.nu0;
   double nu = nu0;
   if (C.has_delta) {
@@ -790,7 +775,9 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   const bool exact_g = C.fp64_widths || near_node(pg, ng);
   const bool exact_l = C.fp64_widths || near_node(pl, nl);
   if (exact_g) {
-    const double p = axis_position(log(fwhm_gauss(nu0, T.dopp[iso])), T.laG, T.slG, ng);
+    // the exact path is rare (positions within 2e-4 of a node): its tables
+    // stay in global memory, where they cost the fill no LDS
+    const double p = axis_position(log(fwhm_gauss(nu0, C.job->dopp[iso])), C.axes->laG, C.axes->slG, ng);
     t.l0 = (int)p;
     t.tg = (float)(p - (double)t.l0);
   } else {
@@ -799,7 +786,7 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
     t.tg = pg - f;
   }
   if (exact_l) {
-    const double p = axis_position(log(fwhm_lorentz_c(C, r.ga, r.gs, r.na, r.ns)), T.laL, T.slL, nl);
+    const double p = axis_position(log(fwhm_lorentz_c(C, r.ga, r.gs, r.na, r.ns)), C.axes->laL, C.axes->slL, nl);
     t.m0 = (int)p;
     t.tl = (float)(p - (double)t.m0);
   } else {
@@ -858,14 +845,14 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 
 template <bool NONEQ, bool PLAIN>
 __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, const int4 it, const LdmDb& db,
-                                          double* acc, LineTables& tab) {
+                                          double* acc) {
   const int M = A.m;
   const int ng = A.ng, nl = A.nl;
   const int W = J.tile_w, Wp = W + LDM_FILL_PAD;
   const long long k_lo = (long long)it.y * W;
   long long k_hi = k_lo + W;
   if (k_hi > J.v) k_hi = J.v;
-  const LineConst C(J, db);
+  const LineConst C(J, A, db);
   const LineCols cols(db);
   const unsigned lo = (unsigned)(J.i0 + it.z), hi = (unsigned)(J.i0 + it.w);  // written by k_tile_bounds
   // Thread t takes lines lo + t, lo + t + T, ...: every load instruction of
@@ -892,7 +879,6 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     double2* a2 = reinterpret_cast<double2*>(acc);
     for (int e = threadIdx.x; e < n2; e += blockDim.x) a2[e] = make_double2(0.0, 0.0);
   }
-  load_tables(J, A, tab);
   __syncthreads();
   const int wlen = (int)(k_hi - k_lo);
   const int sg = nl * Wp;
@@ -904,7 +890,7 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     const unsigned i2 = i + 2 * LDM_FILL_T;
     cols.load<NONEQ, PLAIN>(i2 < hlast ? i2 : hlast, n2);
     LineTerms t;
-    line_terms<NONEQ>(C, cur, cv, tab, ng, nl, t);
+    line_terms<NONEQ>(C, cur, cv, ng, nl, t);
     const int c0 = t.k0 - (int)k_lo;  // column of corner k0 in the tile, in [-1, W-1] if touching
     if (c0 < -1 || c0 > wlen - 1) continue;
     // the 8 corners are base + {0, 1} + {0, nl*Wp} + {0, Wp}
@@ -950,22 +936,21 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   }
 }
 
-__global__ __launch_bounds__(LDM_FILL_T) void k_fill(const LdmJob* __restrict__ jobs,
+__global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
                                                      const int4* __restrict__ items, LdmDb db) {
   extern __shared__ __align__(16) double acc[];
-  __shared__ LineTables tab;
   const int4 it = items[blockIdx.x];
   const LdmJob& J = jobs[it.x];
   const LdmAxes& A = axes[it.x];
   if (A.m == 0 || J.status) return;
   const bool plain = !db.iso && !db.delta && !db.n_self;
   if (J.noneq) {
-    if (plain) fill_tile<true, true>(J, A, it, db, acc, tab);
-    else fill_tile<true, false>(J, A, it, db, acc, tab);
+    if (plain) fill_tile<true, true>(J, A, it, db, acc);
+    else fill_tile<true, false>(J, A, it, db, acc);
   } else {
-    if (plain) fill_tile<false, true>(J, A, it, db, acc, tab);
-    else fill_tile<false, false>(J, A, it, db, acc, tab);
+    if (plain) fill_tile<false, true>(J, A, it, db, acc);
+    else fill_tile<false, false>(J, A, it, db, acc);
   }
 }
 
@@ -1215,13 +1200,10 @@ __global__ __launch_bounds__(256) void k_line_terms(const LdmJob* __restrict__ j
                                                     int job, LdmDb db, long long* __restrict__ k0, int* __restrict__ l0,
                                                     int* __restrict__ m0, float* __restrict__ tv, float* __restrict__ tg,
                                                     float* __restrict__ tl, float* __restrict__ I) {
-  __shared__ LineTables tab;
   const LdmJob& J = jobs[job];
   const LdmAxes& A = axes[job];
-  load_tables(J, A, tab);
-  __syncthreads();
   if (A.m == 0 || J.status) return;
-  const LineConst C(J, db);
+  const LineConst C(J, A, db);
   const LineCols cols(db);
   const long long n = J.i1 - J.i0;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
@@ -1231,11 +1213,11 @@ __global__ __launch_bounds__(256) void k_line_terms(const LdmJob* __restrict__ j
     if (J.noneq) {
       cols.load<true>((unsigned)(J.i0 + q), r);
       iso_vals(J, r.iso, iv);
-      line_terms<true>(C, r, iv, tab, A.ng, A.nl, t);
+      line_terms<true>(C, r, iv, A.ng, A.nl, t);
     } else {
       cols.load((unsigned)(J.i0 + q), r);
       iso_vals(J, r.iso, iv);
-      line_terms(C, r, iv, tab, A.ng, A.nl, t);
+      line_terms(C, r, iv, A.ng, A.nl, t);
     }
     k0[q] = t.k0; l0[q] = t.l0; m0[q] = t.m0;
     tv[q] = t.tv; tg[q] = t.tg; tl[q] = t.tl; I[q] = t.I;
