@@ -211,6 +211,7 @@ struct Req {
   int scale_log2 = 0;
   int bA = 0, bB = 0, bC = 0;  // log2 of the pass A / C column blocks and pass B row blocks
   int n_axp = 0;       // axis partials (items of k_axis)
+  bool sky_whole = false;  // width extremes from the whole list's Pareto sets in k_finalize (no k_axis items)
   size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
 };
@@ -624,6 +625,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.row_inv_f0 = (float*)(work + R.off_rows + (size_t)12 * R.m_cap);
   J.ax_part = (unsigned long long*)(work + R.off_axp);
   J.n_ax_part = R.n_axp;
+  J.sky_whole = R.sky_whole ? 1 : 0;
   J.tw1 = R.tw1;
   J.tw2 = R.tw2;
   J.twl_hi = R.twl_hi;
@@ -753,14 +755,19 @@ int build_layout(radis_ldm_batch* b) {
     const int64_t nl = R.i1 - R.i0;
     {
       // axis items: lines of partial blocks scanned, whole blocks through
-      // their width candidates (LDM_AX_CHUNK == LDM_SKY_BLOCK)
+      // their width candidates (LDM_AX_CHUNK == LDM_SKY_BLOCK); none when
+      // the window is a single-isotopologue list's every line (k_finalize
+      // evaluates the list's Pareto sets instead)
       int np = 0;
+      b->reqs[j].sky_whole = b->db->has_sky && !b->db->has_iso && R.i0 == 0 && R.i1 == b->db->n && R.i1 > R.i0 &&
+                             !(R.r.flags & RADIS_LDM_SCAN_AXIS);
       auto scan = [&](int64_t a, int64_t e) {
         for (int64_t c = a; c < e; c += LDM_AX_CHUNK)
           ax.push_back(make_int4(j, np++, (int)(c - R.i0), (int)std::min<int64_t>(LDM_AX_CHUNK, e - c)));
       };
       const int64_t fb0 = (R.i0 + LDM_SKY_BLOCK - 1) / LDM_SKY_BLOCK, fb1 = R.i1 / LDM_SKY_BLOCK;
-      if (b->db->has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
+      if (R.sky_whole) {
+      } else if (b->db->has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
         scan(R.i0, fb0 * LDM_SKY_BLOCK);
         // at least LDM_SKY_GROUP_MIN blocks per workgroup (each partial costs
         // the reduction a read), and at most about LDM_SKY_WGS workgroups
@@ -1229,6 +1236,11 @@ int radis_ldm_db_upload(radis_ldm_ctx* ctx, const radis_ldm_lines* L, radis_ldm_
         point_skyline(std::move(pts), d, db->h_gsky[d]);
       });
     for (auto& x : tg) x.join();
+    for (int d = 0; d < 4; ++d) {
+      db->dev.gsky_off[d] = (int)sky_vals.size();
+      sky_vals.insert(sky_vals.end(), db->h_gsky[d].begin(), db->h_gsky[d].end());
+    }
+    db->dev.gsky_off[4] = (int)sky_vals.size();
     if (db->has_iso) {
       iso_rng.assign((size_t)nfull * L->n_iso_slots, make_int2(-1, -1));
       for (int64_t i = 0; i < (int64_t)nfull * LDM_SKY_BLOCK; ++i) {
@@ -1569,9 +1581,12 @@ int radis_ldm_batch_info(radis_ldm_batch* b, int i, radis_ldm_request_info* info
   {
     const radis_ldm_db& db = *b->db;
     const int64_t fb0 = (R.i0 + LDM_SKY_BLOCK - 1) / LDM_SKY_BLOCK, fb1 = R.i1 / LDM_SKY_BLOCK;
-    if (db.has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
+    const int dmax = std::log(R.r.Tref / R.r.Tgas) > 0.0 ? 2 : 0;
+    if (R.sky_whole) {
+      info->axis_candidates = db.dev.gsky_off[dmax + 2] - db.dev.gsky_off[dmax];
+      info->axis_scanned = 0;
+    } else if (db.has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
       const int nb1 = db.dev.sky_nb + 1;
-      const int dmax = std::log(R.r.Tref / R.r.Tgas) > 0.0 ? 2 : 0;
       int64_t cand = 0;
       for (int d = dmax; d <= dmax + 1; ++d) cand += db.h_sky_off[(size_t)d * nb1 + fb1] - db.h_sky_off[(size_t)d * nb1 + fb0];
       info->axis_candidates = cand;

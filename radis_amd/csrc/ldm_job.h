@@ -66,6 +66,8 @@ struct LdmDb {
   const float4* sky;
   const int* sky_off;
   int sky_nb;
+  // the whole list's Pareto sets, direction d: sky[gsky_off[d] .. gsky_off[d + 1])
+  int gsky_off[5];
   int n_iso_slots;
   const int2* iso_rng;   // [sky_nb][n_iso_slots]: first/last line of each iso in the block (-1: none)
   const LdmQSpline* qs;  // [LDM_MAX_ISO] partition-function splines (slots in qs_mask), may be null
@@ -132,6 +134,8 @@ struct LdmJob {
   const float2* twl_lo;          // W_L^(e), e < 1024
   unsigned long long* ax_part;   // [n_ax_part][4]: per-item width extremes (wl min, wl max, wg min, wg max) as bits
   int n_ax_part;
+  int sky_whole;                 // 1: the window is a single-isotopologue list's every line: k_finalize takes
+                                 // the Lorentzian width extremes from the list's Pareto sets (no k_axis items)
   // reduction targets and status, initialised by the host every run
   unsigned long long wl_min_bits, wl_max_bits, wg_min_bits, wg_max_bits;
   int status;

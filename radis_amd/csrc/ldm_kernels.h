@@ -394,14 +394,28 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
     if (!(qi > 0x1p-60 && qi < 0x1p+100)) J.i_fast = 0;
   }
   {
-    // reduce the per-item width extremes written by k_axis
+    // reduce the per-item width extremes written by k_axis, or, when the
+    // window is the whole single-isotopologue list, evaluate the list's two
+    // Pareto sets here (the extremes k_axis would find, bit for bit: same
+    // candidates' superset property and same FP64 expression)
     unsigned long long lmin = ~0ull, lmax = 0ull, gmin = ~0ull, gmax = 0ull;
-    for (long long c = threadIdx.x; c < nch; c += blockDim.x) {
-      const unsigned long long* P = part + 4 * c;
-      lmin = P[0] < lmin ? P[0] : lmin;
-      lmax = P[1] > lmax ? P[1] : lmax;
-      gmin = P[2] < gmin ? P[2] : gmin;
-      gmax = P[3] > gmax ? P[3] : gmax;
+    if (J.sky_whole) {
+      const int dmax = J.log_r > 0.0 ? 2 : 0;
+      const int a1 = db.gsky_off[dmax + 1], b1 = db.gsky_off[dmax + 2];
+      for (int c = db.gsky_off[dmax] + threadIdx.x; c < b1; c += blockDim.x) {
+        const float4 q = db.sky[c];
+        const unsigned long long bl = double_to_bits(fwhm_lorentz(J, q.x, q.y, q.z, q.w));
+        if (c < a1) lmax = bl > lmax ? bl : lmax;
+        else lmin = bl < lmin ? bl : lmin;
+      }
+    } else {
+      for (long long c = threadIdx.x; c < nch; c += blockDim.x) {
+        const unsigned long long* P = part + 4 * c;
+        lmin = P[0] < lmin ? P[0] : lmin;
+        lmax = P[1] > lmax ? P[1] : lmax;
+        gmin = P[2] < gmin ? P[2] : gmin;
+        gmax = P[3] > gmax ? P[3] : gmax;
+      }
     }
     for (int off = 32; off > 0; off >>= 1) {
       unsigned long long o;

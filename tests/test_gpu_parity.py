@@ -311,6 +311,35 @@ def test_axis_candidates_equal_full_scan(dev, multi_iso, with_nself):
     db.close()
 
 
+@pytest.mark.parametrize("with_nself", [False, True])
+def test_axis_whole_list_from_pareto_sets(dev, with_nself):
+    """A window holding every line of a single-isotopologue list takes its
+    Lorentzian width extremes from the list's Pareto sets in k_finalize (no
+    k_axis items): bit-identical to the full scan and to NumPy's FP64
+    widths to rounding, for T above, at and below Tref."""
+    lines = _axis_lines(400_000, 44, False, with_nself)
+    d = lines.as_dict()
+    db = engine.LineDatabase(dev, lines)
+    for T, x in [(1500.0, 0.3), (296.0, 0.3), (200.0, 0.0), (900.0, 1.0)]:
+        c = workloads.Conditions(float(d["nu0"][0]) - 1.0, float(d["nu0"][-1]) + 1.0, 0.002, T, 1.3, x)
+        r0 = engine.request_from_conditions(db, c)
+        r1 = engine.request_from_conditions(db, c)
+        r1.flags = int(r1.flags) | _native.SCAN_AXIS
+        b = engine.SpectrumBatch(db, [r0, r1])
+        res = b.results(b.run())
+        i0, i1 = b.info(0), b.info(1)
+        assert i0.axis_scanned == 0 and i0.axis_candidates > 0, (T, x)
+        assert i1.axis_scanned == len(d["nu0"])
+        for k in ("wg_min", "wg_max", "wl_min", "wl_max"):
+            assert getattr(i0, k) == getattr(i1, k), (T, x, k)
+        np.testing.assert_array_equal(res[0], res[1])
+        _, _, wG, wL = oracle.line_parameters(d, T, 1.3, x)
+        np.testing.assert_allclose([i0.wl_min, i0.wl_max], [wL.min(), wL.max()], rtol=1e-15)
+        np.testing.assert_allclose([i0.wg_min, i0.wg_max], [wG.min(), wG.max()], rtol=1e-15)
+        b.close()
+    db.close()
+
+
 def test_grid_index_at_and_just_below_grid_points(dev):
     """Lines on grid points, and 1..64 ulp and 1e-12..1e-6 steps below the
     next one, where NumPy's (1/dw)*(x - w[j]) + j rounds up to j + 1: the
