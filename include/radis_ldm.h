@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RADIS_LDM_ABI_VERSION 4
+#define RADIS_LDM_ABI_VERSION 5
 
 enum {
     RADIS_LDM_OK = 0,
@@ -84,7 +84,9 @@ typedef struct {
     double dxG, dxL;          /* LDM log-steps (RADIS Parameters.dxG/dxL)               */
     double folding_thresh;    /* alias folding threshold (RADIS default 1e-6)           */
     double q_ratio[RADIS_LDM_MAX_ISO]; /* Q(Tref)/Q(Tgas) per iso slot (host evaluates Q) */
-    int64_t fft_len;          /* 0 = planner choice (smallest supported L >= 2*N_v)     */
+    int64_t fft_len;          /* 0 = planner choice: 2*N_v when it has a supported split,
+                                 else the smallest supported L with room for the
+                                 circular-image correction (see RADIS_LDM_PERIOD_L) */
     int64_t n_v;              /* 0 = len(arange(wmin, wmax+wstep, wstep)); else the grid
                                  is wmin + i*((wmin+wstep)-wmin), i < n_v (gpuApp's Nv) */
     double line_lo, line_hi;  /* if line_hi > line_lo: lines with line_lo<=nu0<=line_hi,
@@ -121,6 +123,15 @@ typedef struct {
  * device; q_ratio is then ignored for those slots.  Tgas and Tref must lie
  * in each table's range.  Equilibrium requests only. */
 #define RADIS_LDM_DEVICE_Q 32
+/* The spectrum is the circular convolution at the transform length fft_len
+ * (the reference GPU's semantics at next_fast_len_even(2*N_v),
+ * radis/gpu/gpu.py:22-29, 133-154) instead of RADIS's CPU result at
+ * L = 2*N_v (radis/lbl/broadening.py:1900-1902, 2340-2346).  Without this
+ * flag a transform longer than 2*N_v has its circular images corrected
+ * (k_wrap), so the output is the L = 2*N_v result whatever fft_len is; an
+ * explicit fft_len must then be 2*N_v itself or leave a gap the correction
+ * can span (about 0.5 % of L). */
+#define RADIS_LDM_PERIOD_L 64
 
 /* Planned sizes of a request (filled by radis_ldm_batch_create / _info). */
 typedef struct {
@@ -209,12 +220,12 @@ int radis_ldm_batch_workspace_bytes(const radis_ldm_batch *b, int64_t *bytes);
 void radis_ldm_batch_destroy(radis_ldm_batch *b);
 
 /* Device time per stage (axis, finalize, tile_bounds, fill, fft_a, fft_b,
- * fft_c) in milliseconds, averaged over the runs made with timing enabled
+ * fft_c, wrap) in milliseconds, averaged over the runs made with timing enabled
  * since the previous call; each stage is bracketed by start/stop events
  * (hipExtLaunchKernelGGL) on the launch stream.  A stage whose kernel did not
  * run (tile bounds reused) reports 0.
  * Replaces GPUApplication.get_timestamps (vulkan_compute_lib.py:973-1024). */
-#define RADIS_LDM_N_STAGES 7
+#define RADIS_LDM_N_STAGES 8
 int radis_ldm_batch_enable_timing(radis_ldm_batch *b, int enable);
 int radis_ldm_batch_stage_ms(radis_ldm_batch *b, float *ms_out);
 

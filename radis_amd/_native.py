@@ -13,8 +13,8 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libradis_ldm.so")
 MAX_ISO = 16
-N_STAGES = 7
-STAGE_NAMES = ("axis", "finalize", "tile_bounds", "fill", "fft_a", "fft_b", "fft_c")
+N_STAGES = 8
+STAGE_NAMES = ("axis", "finalize", "tile_bounds", "fill", "fft_a", "fft_b", "fft_c", "wrap")
 
 # status codes (radis_ldm.h)
 OK, EINVAL, EHIP, ENOMEM, ERANGE, EDEVICE = 0, -1, -2, -3, -4, -5
@@ -89,7 +89,8 @@ MIN_RMS = 4
 NONEQ = 8
 SCAN_AXIS = 16
 DEVICE_Q = 32
-ABI_VERSION = 4
+PERIOD_L = 64
+ABI_VERSION = 5
 
 
 class RequestInfo(ctypes.Structure):

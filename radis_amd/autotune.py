@@ -30,6 +30,30 @@ FC_N2 = (256, 512, 1024, 2048)
 TILE_WIDTHS = (16, 32, 64, 128, 256, 512)
 FILL_LDS_MAX = 60 * 1024  # bytes of FP64 tile cells per fill workgroup
 FFT_SLACK = 1.6           # compiled splits considered up to this factor above the smallest generic L
+# a transform longer than 2 N_v needs room for the library's circular-image
+# correction: L >= wrap_floor(2 N_v) (mirrors LDM_WRAP_* of the library)
+WRAP_KMAX = 256
+WRAP_C = 4.0 * 10.0 / (2.0 * 3.141592653589793)
+
+
+def _wrap_bins(L, need):
+    import math
+
+    w = min(L - need, 0.25 * need)
+    return math.ceil(WRAP_C * L / w)
+
+
+def wrap_floor(need):
+    """Smallest transform length above ``need`` = 2 N_v whose circular
+    images the library can correct."""
+    import math
+
+    L = math.ceil(need * WRAP_KMAX / (WRAP_KMAX - WRAP_C))
+    while L <= 40 * need and _wrap_bins(L, need) > WRAP_KMAX:
+        L += 1
+    return L
+
+
 FILL_STAGES = ("tile_bounds", "fill")
 FFT_STAGES = ("fft_a", "fft_b", "fft_c")
 
@@ -112,8 +136,9 @@ def lookup(db, req):
 def apply(db, req):
     """Set a cached schedule on ``req`` in place; returns whether one was
     found.  Requests of one size class can differ slightly in N_v: the FFT
-    schedule is applied only if its length covers 2 N_v of this request (the
-    tile width always)."""
+    schedule is applied only if its length is admissible for this request's
+    2 N_v (2 N_v itself, or at least wrap_floor(2 N_v)); the tile width
+    always."""
     c = _load()
     if not c:
         return False
@@ -122,7 +147,8 @@ def apply(db, req):
     if not e:
         return False
     req.tile_w = int(e["tile_w"])
-    if int(e["fft_len"]) >= 2 * int(info.n_v):
+    need, L = 2 * int(info.n_v), int(e["fft_len"])
+    if L == need or L >= wrap_floor(need):
         req.fft_len, req.fft_n1 = int(e["fft_len"]), int(e["fft_n1"])
         req.flags = (int(req.flags) | _generic_flag()) if e["generic"] else (int(req.flags) & ~_generic_flag())
     return True
@@ -142,13 +168,18 @@ def _smooth16(limit):
 
 def fft_candidates(nv, n_generic=4):
     """(fft_len, n1, generic) candidates for a grid of nv points: the smallest
-    7-smooth length with up to ``n_generic`` splits (run-time-sized passes),
-    and every compiled split with 2 nv <= L <= FFT_SLACK * that length."""
+    admissible 7-smooth length (2 nv itself, else at least wrap_floor(2 nv))
+    with up to ``n_generic`` splits (run-time-sized passes), and every
+    admissible compiled split up to FFT_SLACK times that length."""
     need = 2 * nv
+    lo = wrap_floor(need)
     c2 = _smooth16(8192)
     gen = []
     for n1 in _smooth16(1024):
-        n2min = -(-need // n1)
+        if need % n1 == 0 and need // n1 in c2:
+            gen.append((need, n1))
+            continue
+        n2min = -(-lo // n1)
         n2 = next((v for v in c2 if v >= n2min), None)
         if n2 is not None:
             gen.append((n1 * n2, n1))
@@ -160,7 +191,7 @@ def fft_candidates(nv, n_generic=4):
     for n1 in FC_N1:
         for n2 in FC_N2:
             L = n1 * n2
-            if need <= L <= FFT_SLACK * Lmin:
+            if (L == need or lo <= L) and L <= FFT_SLACK * Lmin:
                 out.append((L, n1, False))
     return out
 

@@ -20,6 +20,7 @@
 #include "ldm_kernels.h"
 #include "ldm_fftc.h"
 #include "ldm_slit.h"
+#include "ldm_wrap.h"
 
 namespace {
 
@@ -187,7 +188,9 @@ struct FftPlan {
   int64_t L = 0;
   int n1 = 0, n2 = 0;
   bool fc = false;  // compile-time-sized passes (ldm_fftc.h) for this n1, n2
-  std::vector<int> r1, r2;
+  int nr1 = 0, nr2 = 0;
+  int r1[LDM_MAX_RAD] = {0}, r2[LDM_MAX_RAD] = {0};
+  int wrap_k = 0;   // bins of the circular-image correction (0: L is the spectrum's period)
 };
 
 // Sizes with compile-time-sized passes: n1 (passes A and C) and n2 (pass B).
@@ -206,13 +209,14 @@ struct Req {
   int64_t i0 = 0, i1 = 0;
   int m_need = 0;      // bound for the current conditions
   int m_cap = 0;       // allocated rows
+  int wrap_rows = 0;   // L-axis nodes of the image-correction table (N_L <= M/2: N_G >= 2)
   int tile_w = 0, n_tiles = 0;
   int64_t stride = 0;
   int scale_log2 = 0;
   int bA = 0, bB = 0, bC = 0;  // log2 of the pass A / C column blocks and pass B row blocks
   int n_axp = 0;       // axis partials (items of k_axis)
   bool sky_whole = false;  // width extremes from the whole list's Pareto sets in k_finalize (no k_axis items)
-  size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0;
+  size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0, off_wrap = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr;
 };
 
@@ -241,7 +245,7 @@ struct radis_ldm_batch {
   size_t tw_bytes = 0;
   int4* items_d = nullptr;
   size_t items_bytes = 0;
-  int n_axis = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
+  int n_axis = 0, n_wrap = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
   // launches of the FFT passes: items of one kernel variant are contiguous;
   // cls = the compiled size (n1 for A and C, n2 for B), 0 = generic (pass B:
   // 0 = generic with small tiles, -1 = generic with large tiles)
@@ -282,15 +286,36 @@ std::vector<int> smooth16_list(int maxv) {
   return out;
 }
 
-bool radices_of(int n, std::vector<int>& r) {
-  r.clear();
+bool radices_of(int n, int* r, int& cnt) {
+  cnt = 0;
   for (int p : {16, 8, 4, 2, 7, 5, 3}) {
     while (n % p == 0) {
-      r.push_back(p);
+      if (cnt == LDM_MAX_RAD) return false;
+      r[cnt++] = p;
       n /= p;
     }
   }
-  return n == 1 && (int)r.size() <= LDM_MAX_RAD;
+  return n == 1;
+}
+
+// Bins of the circular-image correction for a transform of length L of a
+// spectrum with period need = 2 N_v (LDM_WRAP_* in ldm_job.h): 0 when
+// L == need, -1 when the window over the gap would need more than
+// LDM_WRAP_KMAX bins (a gap narrower than ~2.5 % of L, or L above ~20 N_v).
+int wrap_bins(int64_t L, int64_t need) {
+  if (L == need) return 0;
+  if (L < need) return -1;
+  const double w = std::min((double)(L - need), 0.25 * (double)need);
+  const double k = std::ceil(LDM_WRAP_CK * LDM_WRAP_NSIG / (2 * 3.14159265358979323846) * (double)L / w);
+  return k <= LDM_WRAP_KMAX ? (int)k : -1;
+}
+
+// Smallest transform length above need whose gap is wide enough.
+int64_t wrap_floor(int64_t need) {
+  const double c = LDM_WRAP_CK * LDM_WRAP_NSIG / (2 * 3.14159265358979323846);
+  int64_t L = (int64_t)std::ceil((double)need * LDM_WRAP_KMAX / (LDM_WRAP_KMAX - c));
+  while (L <= 40 * need && wrap_bins(L, need) < 0) ++L;
+  return L;
 }
 
 // Search L = n1*n2 >= 2*N_v (or == want) with 16 | n1 <= 1024 and
@@ -324,12 +349,30 @@ int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int
   return bestL;
 }
 
-int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow_fc = true) {
+// Plan the transform of a spectrum of period need = 2 N_v.  With `period`
+// (RADIS_LDM_PERIOD_L) the transform length is the spectrum's period, any
+// L >= need; otherwise L is need itself or at least wrap_floor(need), and
+// k_wrap corrects the images (the result is RADIS's at L = 2 N_v).
+int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow_fc = true, bool period = false) {
   const int64_t need = 2 * nv;
   if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);
+  if (want > 0 && !period && wrap_bins(want, need) < 0)
+    return fail(RADIS_LDM_ERANGE,
+                "fft_len %lld is too close to 2*N_v = %lld for the circular-image correction (need L = 2*N_v or "
+                "L >= %lld)", (long long)want, (long long)need, (long long)wrap_floor(need));
+  // smallest admissible length among the candidates of `search` (L == need
+  // itself, else from `lo` on)
+  const int64_t lo = period ? need : wrap_floor(need);
+  auto pick = [&](int n2_max, int& b1, int& b2) -> int64_t {
+    if (want > 0) return search_fft(need, want, n2_max, b1, b2, only_n1);
+    int e1 = 0, e2 = 0;
+    const int64_t exact = search_fft(need, need, n2_max, e1, e2, only_n1);
+    if (exact > 0) { b1 = e1; b2 = e2; return exact; }
+    return search_fft(lo, 0, n2_max, b1, b2, only_n1);
+  };
   int a1 = 0, a2 = 0, s1 = 0, s2 = 0;
-  const int64_t any = search_fft(need, want, N2_MAX, a1, a2, only_n1);
-  const int64_t small = search_fft(need, want, LDM_CAP_B, s1, s2, only_n1);  // pass B with many small tiles
+  const int64_t any = pick(N2_MAX, a1, a2);
+  const int64_t small = pick(LDM_CAP_B, s1, s2);  // pass B with many small tiles
   if (any < 0) {
     if (want > 0)
       return fail(RADIS_LDM_ERANGE, "fft_len %lld is not n1*n2 with 16|n1<=%d, 16|n2<=%d and 7-smooth cofactors",
@@ -342,7 +385,7 @@ int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow
   out.n1 = use_small ? s1 : a1;
   out.n2 = use_small ? s2 : a2;
   out.fc = false;
-  // compiled split: smallest L = n1*n2 >= need (or == want), then n1 closest to 512
+  // compiled split: smallest admissible L = n1*n2 (or == want), then n1 closest to 512
   int64_t bestc = -1;
   int c1 = 0, c2 = 0;
   for (int n1 : FC_N1) {
@@ -350,7 +393,7 @@ int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow
     if (only_n1 && n1 != only_n1) continue;
     for (int n2 : FC_N2) {
       const int64_t L = (int64_t)n1 * n2;
-      if (want > 0 ? L != want : L < need) continue;
+      if (want > 0 ? L != want : (L != need && L < lo)) continue;
       if (bestc < 0 || L < bestc || (L == bestc && std::abs(n1 - 512) < std::abs(c1 - 512))) {
         bestc = L; c1 = n1; c2 = n2;
       }
@@ -362,8 +405,13 @@ int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow
     out.n2 = c2;
     out.fc = true;
   }
-  if (!radices_of(out.n1, out.r1) || !radices_of(out.n2, out.r2))
+  if (!radices_of(out.n1, out.r1, out.nr1) || !radices_of(out.n2, out.r2, out.nr2))
     return fail(RADIS_LDM_ERANGE, "radix factorisation failed");
+  out.wrap_k = period ? 0 : wrap_bins(out.L, need);
+  if (out.wrap_k < 0)
+    return fail(RADIS_LDM_ERANGE,
+                "grid of %lld points is too short for the circular-image correction of the smallest transform "
+                "(L = %lld); request RADIS_LDM_PERIOD_L", (long long)nv, (long long)out.L);
   return 0;
 }
 
@@ -597,8 +645,8 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.single_iso = db.has_iso ? 0 : 1;
   J.n1 = R.plan.n1;
   J.n2 = R.plan.n2;
-  J.nrad1 = (int)R.plan.r1.size();
-  J.nrad2 = (int)R.plan.r2.size();
+  J.nrad1 = R.plan.nr1;
+  J.nrad2 = R.plan.nr2;
   for (int i = 0; i < J.nrad1; ++i) J.rad1[i] = R.plan.r1[i];
   for (int i = 0; i < J.nrad2; ++i) J.rad2[i] = R.plan.r2[i];
   J.bA = R.bA;
@@ -625,6 +673,10 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.row_inv_f0 = (float*)(work + R.off_rows + (size_t)12 * R.m_cap);
   J.ax_part = (unsigned long long*)(work + R.off_axp);
   J.n_ax_part = R.n_axp;
+  J.wrap_k = R.plan.wrap_k;
+  J.wrap_rows = R.wrap_rows;
+  J.wrap_h = R.plan.wrap_k ? (float*)(work + R.off_wrap) : nullptr;
+  J.wrap_ls = 2 * R.nv;
   J.sky_whole = R.sky_whole ? 1 : 0;
   J.tw1 = R.tw1;
   J.tw2 = R.tw2;
@@ -675,6 +727,9 @@ int layout_request(Req& R, size_t& total) {
   R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
   // axis partials: at most one per chunk, plus the two partial blocks at the window edges
   R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2) * 32, 256);
+  // circular-image correction: wrap_k floats per L-axis node (N_L <= M/2, as N_G >= 2)
+  R.wrap_rows = R.plan.wrap_k ? std::max(1, std::min(R.m_cap / 2, LDM_MAX_AX)) : 0;
+  R.off_wrap = total; total = align_up(total + (size_t)R.wrap_rows * R.plan.wrap_k * 4, 256);
   return 0;
 }
 
@@ -748,7 +803,7 @@ int build_layout(radis_ldm_batch* b) {
     R.twl_lo = R.twl_hi + (size_t)((R.plan.L + 1023) >> LDM_TW_LO_BITS);
   }
   // --- work items (FFT passes grouped by kernel variant)
-  std::vector<int4> ax, fi;
+  std::vector<int4> ax, fi, wr;
   std::map<int, std::vector<int4>> ma, mb, mc;
   for (int j = 0; j < b->n; ++j) {
     const Req& R = b->reqs[j];
@@ -781,6 +836,7 @@ int build_layout(radis_ldm_batch* b) {
       b->reqs[j].n_axp = np;
     }
     (void)nl;
+    for (int m = 0; m < R.wrap_rows; m += 2) wr.push_back(make_int4(j, m / 2, 0, 0));
     for (int t = 0; t < R.n_tiles; ++t) fi.push_back(make_int4(j, t, 0, 0));
     const int pcap = (R.m_cap + 1) / 2;
     const int nA = R.plan.n2 >> R.bA, nC = R.plan.n2 >> R.bC;
@@ -804,12 +860,13 @@ int build_layout(radis_ldm_batch* b) {
   flatten(mb, fb, b->grp_b);
   flatten(mc, fc, b->grp_c);
   b->n_axis = (int)ax.size();
+  b->n_wrap = (int)wr.size();
   b->n_fill = (int)fi.size();
   b->n_a = (int)fa.size();
   b->n_b = (int)fb.size();
   b->n_c = (int)fc.size();
   std::vector<int4> all;
-  for (auto* v : {&ax, &fi, &fa, &fb, &fc}) all.insert(all.end(), v->begin(), v->end());
+  for (auto* v : {&ax, &wr, &fi, &fa, &fb, &fc}) all.insert(all.end(), v->begin(), v->end());
   if (all.empty()) all.push_back(make_int4(0, 0, 0, 0));
   b->items_bytes = all.size() * sizeof(int4);
   b->items_d = (int4*)dev_alloc(b->ctx, b->items_bytes);
@@ -833,7 +890,8 @@ int plan_one(const radis_ldm_db& db, const radis_ldm_request& rq, Req& R) {
     return fail(RADIS_LDM_EINVAL, "fft_n1 %d: need 16 | fft_n1 <= %d", rq.fft_n1, N1_MAX);
   if (rq.tile_w && (rq.tile_w < MIN_TILE_W || rq.tile_w > MAX_TILE_W || (rq.tile_w & (rq.tile_w - 1))))
     return fail(RADIS_LDM_EINVAL, "tile_w %d: need a power of 2 in [%d, %d]", rq.tile_w, MIN_TILE_W, MAX_TILE_W);
-  rc = plan_fft(R.nv, rq.fft_len, R.plan, rq.fft_n1, !(rq.flags & RADIS_LDM_GENERIC_FFT));
+  rc = plan_fft(R.nv, rq.fft_len, R.plan, rq.fft_n1, !(rq.flags & RADIS_LDM_GENERIC_FFT),
+                (rq.flags & RADIS_LDM_PERIOD_L) != 0);
   if (rc) return rc;
   const bool lw = rq.line_hi > rq.line_lo;
   const double lo = lw ? rq.line_lo : rq.wmin, hi = lw ? rq.line_hi : rq.wmax;
@@ -886,8 +944,8 @@ void launch(radis_ldm_batch* b, bool timed, int slot, int kid, K kernel, dim3 gr
   launch_part(b, timed, slot, kid, true, true, kernel, grid, block, shmem, s, args...);
 }
 
-// stage of each kernel id: axis, finalize, tile bounds, fill, fft_a, fft_b, fft_c
-const int KERNEL_STAGE[RADIS_LDM_N_STAGES] = {0, 1, 2, 3, 4, 5, 6};
+// stage of each kernel id: axis, finalize, tile bounds, fill, fft_a, fft_b, fft_c, wrap
+const int KERNEL_STAGE[RADIS_LDM_N_STAGES] = {0, 1, 2, 3, 4, 5, 6, 7};
 
 // The kernel sequence of one run (what a graph captures).
 int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool capturing = false) {
@@ -899,6 +957,8 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
   if (b->n_axis) launch(b, timed, slot, 0, k_axis, dim3(b->n_axis), dim3(256), 0, s, b->jobs_d, items, db.dev);
   launch(b, timed, slot, 1, k_finalize, dim3(b->n), dim3(256), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
   items += b->n_axis;
+  if (b->n_wrap) launch(b, timed, slot, 7, k_wrap, dim3(b->n_wrap), dim3(256), 0, s, cj, ca, items);
+  items += b->n_wrap;
   if (b->n_fill) {
     // Tile line bounds depend on the grid and, through the pressure-shift
     // margin, on p; without a shift column they are fixed per layout.
@@ -1457,6 +1517,11 @@ int radis_ldm_batch_set_request(radis_ldm_batch* b, int i, const radis_ldm_reque
       req->line_lo != R.r.line_lo || req->line_hi != R.r.line_hi || req->fft_n1 != R.r.fft_n1 ||
       req->tile_w != R.r.tile_w)
     return fail(RADIS_LDM_EINVAL, "set_request cannot change the grid; create a new batch");
+  // flags fixed by the plan and the work items (FFT variant, axis items, period)
+  const int plan_flags = RADIS_LDM_GENERIC_FFT | RADIS_LDM_SCAN_AXIS | RADIS_LDM_PERIOD_L;
+  if ((req->flags ^ R.r.flags) & plan_flags)
+    return fail(RADIS_LDM_EINVAL, "set_request cannot change RADIS_LDM_GENERIC_FFT, _SCAN_AXIS or _PERIOD_L; "
+                                  "create a new batch");
   int rc = validate_request(*req);
   if (rc) return rc;
   rc = wait_pending(b);

@@ -411,6 +411,22 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
     u0[s][0] = (float)ks * J.inv_Lws;                    // q = 0, 1, ..., RL/2-1
     u0[s][1] = (float)(L / SH::RL - ks) * J.inv_Lws;     // q = RL-1, RL-2, ..., RL/2
   }
+  // bins below the circular-image cutoff (kk < wrap_k <= N1, k_wrap): with
+  // k = k1 + N1 k2 only k2 = 0 (kk = k1) and k2 = N2 - 1 (kk = N1 - k1)
+  // qualify.  The first is slot (0, 0) of thread b < B (t >> LB == 0); the
+  // second is slot (s, RL - 1) of the thread with t >> LB == NSL - 1.
+  // w_lo / w_hi hold those bins (-1: none) and w_s the second one's s.
+  const int WK = J.wrap_k;
+  int w_lo = -1, w_hi = -1, w_s = 0;
+  if (WK) {
+    if ((int)threadIdx.x < B && k1_0 + (int)threadIdx.x < WK) w_lo = k1_0 + threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < SH::ITL; ++s) {
+      const int t = threadIdx.x + s * T;
+      const int kk = N1 - (k1_0 + (t & (B - 1)));
+      if ((t >> LB) == NSL - 1 && kk < WK) { w_hi = kk; w_s = s; }
+    }
+  }
   float gcache[SH::ITL][SH::RL];
   int g_l = -1;
   auto gauss_fill = [&](int r) {
@@ -446,6 +462,16 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
             const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xrs[s][q]) : ((float)n * J.half_inv_ws + xs[s][q]);
             F[s][q] += __expf(-(a * xa * xa) - bco * xa);
           }
+    }
+    if (w_lo >= 0 || w_hi >= 0) {
+      const float* Hm = J.wrap_h + (r % NLr) * WK;
+      if (w_lo >= 0) F[0][0] += Hm[w_lo];
+      if (w_hi >= 0) {
+        const float hv = Hm[w_hi];
+#pragma unroll
+        for (int s = 0; s < SH::ITL; ++s)
+          if (s == w_s) F[s][SH::RL - 1] += hv;
+      }
     }
     const float f0 = gld(J.row_inv_f0, 4u * (unsigned)r);
 #pragma unroll

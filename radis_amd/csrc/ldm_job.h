@@ -26,6 +26,22 @@
 static_assert(LDM_AX_CHUNK == LDM_SKY_BLOCK, "a partial block is one scan chunk");
 #define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
 #define LDM_NFOLD_MAX 100000
+// Circular-image correction (k_wrap): a transform of length L' > 2 N_v
+// reproduces RADIS's L = 2 N_v result when the lineshape FT of every LDM row
+// gains, at its lowest K bins, the DFT (length L') of the difference of the
+// L- and L'-periodised Lorentzian over offsets |d| <= N_v, continued
+// smoothly to zero over the gap of L' - 2 N_v offsets no output reads: an
+// erfc window of width w = min(gap, N_v / 2) and sigma = w / LDM_WRAP_NSIG.
+// Its spectrum is a Gaussian of width L'/(2 pi sigma) bins, kept up to
+// LDM_WRAP_CK of those widths:
+// K = ceil(LDM_WRAP_CK * LDM_WRAP_NSIG / (2 pi) * L' / w) <= LDM_WRAP_KMAX
+// (so gap >= ~2.5 % of L', and L' <= ~20 N_v).  K <= 256 <= N1 of every
+// compiled split keeps the corrected bins at k2 = 0 and k2 = N2 - 1 of the
+// four-step order.  The table is computed from LDM_WRAP_LC >= 8 K samples.
+#define LDM_WRAP_NSIG 10.0
+#define LDM_WRAP_CK 4.0
+#define LDM_WRAP_KMAX 256
+#define LDM_WRAP_LC 2048
 
 // device status codes (LdmJob::status)
 #define LDM_ST_OK 0
@@ -133,6 +149,10 @@ struct LdmJob {
   const float2* twl_hi;          // W_L^(h*1024)
   const float2* twl_lo;          // W_L^(e), e < 1024
   unsigned long long* ax_part;   // [n_ax_part][4]: per-item width extremes (wl min, wl max, wg min, wg max) as bits
+  float* wrap_h;                 // [wrap_rows][wrap_k]: image correction of the lineshape FT at bins kk < wrap_k, per L-axis node
+  int wrap_k;                    // 0: the transform length is the period (L = 2 N_v, or the request asked for L's own period)
+  int wrap_rows;                 // L-axis nodes the table has room for
+  long long wrap_ls;             // the period the spectrum is computed for (2 N_v)
   int n_ax_part;
   int sky_whole;                 // 1: the window is a single-isotopologue list's every line: k_finalize takes
                                  // the Lorentzian width extremes from the list's Pareto sets (no k_axis items)

@@ -1025,22 +1025,12 @@ __global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_a(const LdmJob* __restrict
   }
 }
 
-// Same, from the bin's frequency x = kk/(L wstep) and xr = (L/2 - kk)/(L wstep)
-// (both computed by the caller once per bin).
-__device__ __forceinline__ float lineshape_ft_x(const LdmJob& J, int r, float x, float xr) {
-  const float a = J.row_a[r], bco = J.row_b[r];
-  float v = __expf(-(a * x * x) - bco * x);
-  const int nf = J.nfold[r];
-  for (int n = 1; n <= nf; ++n) {
-    const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xr) : ((float)n * J.half_inv_ws + x);
-    v += __expf(-(a * xa * xa) - bco * xa);
-  }
-  return v * J.row_inv_f0[r];
-}
-
-// Folded, normalised FT lineshape of LDM row r at bin k (|k| folded to
-// [0, L/2]); radis/lbl/broadening.py:1895-1926.
-__device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, long long kk) {
+// Folded, normalised FT lineshape of LDM row r (N_L = nl) at bin k (|k|
+// folded to [0, L/2]); radis/lbl/broadening.py:1895-1926.  Bins below
+// wrap_k add the circular-image correction of the row's L-axis node
+// (k_wrap) before the normalisation, as the image terms belong to the
+// kernel that RADIS divides by its folded value at x = 0.
+__device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, int nl, long long kk) {
   const float a = J.row_a[r], bco = J.row_b[r];
   const float x = (float)kk * J.inv_Lws;
   float v = __expf(-(a * x * x) - bco * x);
@@ -1052,6 +1042,7 @@ __device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, long long 
       v += __expf(-(a * xa * xa) - bco * xa);
     }
   }
+  if (kk < J.wrap_k) v += J.wrap_h[(r % nl) * J.wrap_k + (int)kk];
   return v * J.row_inv_f0[r];
 }
 
@@ -1074,7 +1065,7 @@ __global__ __launch_bounds__(T, 768 / T) void k_fft_b(const LdmJob* __restrict__
   const int N1 = J.n1, N2 = J.n2, bs = J.bB, B = 1 << bs;
   const int k1_0 = it.y * B;
   const long long L = J.L;
-  const int P = axes[it.x].p, M = axes[it.x].m;
+  const int P = axes[it.x].p, M = axes[it.x].m, NLr = axes[it.x].nl;
   const int total = N2 << bs;
   constexpr int NACC = CAP / T;
   float2 acc[NACC];
@@ -1122,8 +1113,8 @@ __global__ __launch_bounds__(T, 768 / T) void k_fft_b(const LdmJob* __restrict__
         const int k2 = e >> bs, b = e & (B - 1);
         const long long k = (long long)(k1_0 + b) + (long long)N1 * k2;
         const long long kk = (k <= L - k) ? k : (L - k);
-        const float fa = lineshape_ft(J, ra, kk);
-        const float fb = (rb < M) ? lineshape_ft(J, rb, kk) : 0.0f;
+        const float fa = lineshape_ft(J, ra, NLr, kk);
+        const float fb = (rb < M) ? lineshape_ft(J, rb, NLr, kk) : 0.0f;
         const float2 z = buf[pad_idx(e)];
         acc[s].x += z.x * fa + z.y * fb;
         acc[s].y += z.y * fa - z.x * fb;

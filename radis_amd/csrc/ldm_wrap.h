@@ -1,0 +1,107 @@
+// k_wrap: the circular-image correction that makes a transform of length
+// L' > 2 N_v return RADIS's spectrum at L = 2 N_v (radis/lbl/broadening.py:
+// 1900-1902 rfftfreq(2*len(w)), 2340-2346 the LDM of 2*len(w) rows).
+//
+// With the folded FT templates, the real-space kernel of an LDM row at
+// transform length L is the L-periodised, grid-sampled Voigt profile.  An
+// output j < N_v takes LDM column k' <= N_v through that kernel at offset
+// d = j - k' in [-N_v, N_v]; periods L and L' differ only in which periodic
+// images fall there, all at |d + nL| >= N_v grid steps from the centre,
+// where the Gaussian part is gone (relative weight (sigma_G / N_v)^2).  So
+//   g_L(d) - g_L'(d) = P_L(d) - P_L'(d) = c(d)   for |d| <= N_v,
+// with P_L the L-periodised sampled Lorentzian of the row's L-axis node,
+//   P_L(d) = (1/L) sinh(a) / (cosh(a) - cos(2 pi d / L)),  a = 2 pi gamma / L
+// (gamma = HWHM in grid steps; the Fourier series of the periodised
+// Lorentzian).  The L' - 2 N_v offsets past N_v are never read; there c is
+// taken smoothly to zero, c_ext(d) = phi(d) c(d) + phi(L' - d) c(L' - d),
+// phi = 1 on [0, N_v], an erfc step over the next w = min(L' - 2 N_v, N_v/2)
+// offsets (c is smooth there: its nearest singular peak is at d = 2 N_v),
+// sigma = w / LDM_WRAP_NSIG.  c_ext is even, so its DFT H_m is real, and it
+// is negligible beyond K = wrap_k bins: pass B adds H_m[kk] to the
+// lineshape FT of every row on node m at its bins kk < K.  H_m comes from
+// LDM_WRAP_LC samples of c_ext (the sampled sum equals the full one up to
+// aliases of H_m beyond LDM_WRAP_LC - K, far below 1e-9 of it).
+//
+// One workgroup per (request, pair of L-axis nodes): the two real sequences
+// are the real and imaginary parts of one complex transform.
+#pragma once
+#include "ldm_fftc.h"
+
+struct WrapNode {
+  double omE, omE2, E;  // 1 - e^-a, 1 - e^-2a, e^-a for the period's a = 2 pi gamma / period
+};
+
+__device__ __forceinline__ WrapNode wrap_node(double gamma, double period) {
+  const double a = 6.283185307179586 * gamma / period;
+  WrapNode w;
+  w.omE = -expm1(-a);
+  w.omE2 = -expm1(-2.0 * a);
+  w.E = exp(-a);
+  return w;
+}
+
+// P(d) of one period in the overflow-free form (1/L)(1 - E^2) / ((1 - E)^2 +
+// 4 E sin^2(pi d / L)) (sinh and cosh multiplied by 2 e^-a)
+__device__ __forceinline__ double lor_periodic(const WrapNode& w, double d, double inv_period) {
+  const double s = sinpi(d * inv_period);
+  return inv_period * w.omE2 / (w.omE * w.omE + 4.0 * w.E * s * s);
+}
+
+__global__ __launch_bounds__(256) void k_wrap(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
+                                              const int4* __restrict__ items) {
+  constexpr int LC = LDM_WRAP_LC, T = 256;
+  __shared__ float2 buf[LdsSize<LC>::value];
+  __shared__ float2 s_tw[LC];
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  const LdmAxes& A = axes[it.x];
+  const int K = J.wrap_k;
+  const int nl = A.nl;
+  const int ma = 2 * it.y, mb = ma + 1;
+  if (J.status || K == 0 || ma >= nl) return;
+  const bool has_b = mb < nl;
+  // forward table W_LC^e
+  for (int e = threadIdx.x; e < LC; e += T) {
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)LC, &sn, &cs);
+    s_tw[e] = make_float2((float)cs, (float)sn);
+  }
+  const double Lp = (double)J.L, Ls = (double)J.wrap_ls;
+  const double nv = 0.5 * Ls;
+  const double w = fmin(Lp - Ls, 0.5 * nv);
+  const double d_mid = nv + 0.5 * w;                                    // centre of the step
+  const double isig = LDM_WRAP_NSIG / (w * 1.4142135623730951);         // 1 / (sigma sqrt 2)
+  const double z_cut = 7.0 / 1.4142135623730951;                         // phi < 1e-12 beyond
+  const double inv_s = 1.0 / Ls, inv_p = 1.0 / Lp;
+  // HWHM of the two nodes in grid steps: FWHM / (2 wstep)
+  const double h = 0.5 / J.wstep;
+  const double ga = exp(A.laL[ma]) * h, gb = has_b ? exp(A.laL[mb]) * h : ga;
+  const WrapNode sa = wrap_node(ga, Ls), pa = wrap_node(ga, Lp);
+  const WrapNode sb = wrap_node(gb, Ls), pb = wrap_node(gb, Lp);
+  // phi(x) c(x) for both nodes (x in [0, L'))
+  auto side = [&](double x, double& va, double& vb) {
+    const double z = (x - d_mid) * isig;
+    if (z >= z_cut) return;
+    const double phi = 0.5 * erfc(z);
+    va += phi * (lor_periodic(sa, x, inv_s) - lor_periodic(pa, x, inv_p));
+    if (has_b) vb += phi * (lor_periodic(sb, x, inv_s) - lor_periodic(pb, x, inv_p));
+  };
+  const double step = Lp / (double)LC;
+  for (int j = threadIdx.x; j < LC; j += T) {
+    const double d = (double)j * step;
+    double ca = 0.0, cb = 0.0;
+    side(d, ca, cb);
+    side(Lp - d, ca, cb);
+    buf[pad_idx(j)] = make_float2((float)ca, (float)cb);
+  }
+  __syncthreads();
+  fc_fft<LC, 1, T, true>(buf, s_tw);
+  // sum over the L' integer offsets = (L' / LC) * the sampled sum
+  const float sc = (float)step;
+  float* H = J.wrap_h;
+  for (int k = threadIdx.x; k < K; k += T) {
+    const float2 z = buf[pad_idx(k)];
+    H[ma * K + k] = z.x * sc;
+    if (has_b) H[mb * K + k] = z.y * sc;
+  }
+}

@@ -238,7 +238,8 @@ class LineDatabase:
 def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_fraction,
                  dxG=DXG_DEFAULT, dxL=DXL_DEFAULT, Tref=296.0, folding_thresh=FOLDING_THRESH_DEFAULT,
                  fft_len=0, q_ratio=None, n_v=0, line_window=None, fp64_widths=False, fft_n1=0, tile_w=0,
-                 generic_fft=False, optimization="simple", Tvib=None, Trot=None, device_q=None):
+                 generic_fft=False, optimization="simple", Tvib=None, Trot=None, device_q=None,
+                 period_fft_len=False):
     """Fill a ``radis_ldm_request``; Q(Tref)/Q(T) comes from the database's
     partition function unless ``q_ratio`` is given.  ``n_v`` fixes the grid
     length (default: len(arange(wmin, wmax+wstep, wstep))); ``line_window``
@@ -252,7 +253,11 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     ``Ttrans = Trot`` by default, radis/lbl/factory.py:1685-1687).
     ``device_q`` takes Q(Tref)/Q(T) from the database's partition tables,
     evaluated on the device (default: when no ``q_ratio`` is given and the
-    tables cover every isotopologue slot of an equilibrium request)."""
+    tables cover every isotopologue slot of an equilibrium request).
+    The spectrum is RADIS's at FFT length 2 N_v whatever transform length
+    runs (the library corrects the circular images of a longer one);
+    ``period_fft_len`` instead returns the circular result at the transform
+    length itself (the reference GPU's semantics)."""
     if optimization not in ("simple", "min-RMS"):
         raise ValueError(f"optimization must be 'simple' or 'min-RMS', not {optimization!r}")
     r = _native.Request()
@@ -280,7 +285,7 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     r.n_v = int(n_v or 0)
     r.flags = ((_native.FP64_WIDTHS if fp64_widths else 0) | (_native.GENERIC_FFT if generic_fft else 0)
                | (_native.MIN_RMS if optimization == "min-RMS" else 0) | (_native.NONEQ if noneq else 0)
-               | (_native.DEVICE_Q if device_q else 0))
+               | (_native.DEVICE_Q if device_q else 0) | (_native.PERIOD_L if period_fft_len else 0))
     if noneq:
         r.Tvib, r.Trot = float(Tvib), float(Trot)
     r.fft_n1, r.tile_w = int(fft_n1 or 0), int(tile_w or 0)
@@ -293,10 +298,10 @@ def make_request(db: LineDatabase, wmin, wmax, wstep, Tgas, pressure_bar, mole_f
     return r
 
 
-def request_from_conditions(db, cond, fft_len=0, fp64_widths=False):
+def request_from_conditions(db, cond, fft_len=0, fp64_widths=False, period_fft_len=False):
     return make_request(db, cond.wmin, cond.wmax, cond.wstep, cond.Tgas, cond.pressure_bar,
                         cond.mole_fraction, cond.dxG, cond.dxL, cond.Tref, fft_len=fft_len,
-                        fp64_widths=fp64_widths)
+                        fp64_widths=fp64_widths, period_fft_len=period_fft_len)
 
 
 def plan_grid(wmin, wmax, wstep):
@@ -345,9 +350,23 @@ class SpectrumBatch:
         return int(v.value)
 
     # -- conditions -------------------------------------------------------------
+    # request fields fixed by the plan: the schedule and the flags that pick
+    # the FFT variant, the axis work items and the period
+    _PLAN_FLAGS = _native.GENERIC_FFT | _native.SCAN_AXIS | _native.PERIOD_L
+
     def set_request(self, i, req):
-        check(self._lib.radis_ldm_batch_set_request(self._h, int(i), byref(req)))
-        self._reqs[i] = req
+        """New conditions for request i (same grid and line window).  The
+        schedule of the batch's request stays: fft_len, fft_n1, tile_w and
+        the FFT-variant, axis-scan and period flags are taken from it, so a
+        request built by make_request (which may apply a different tuned
+        schedule for the new conditions' size class) keeps the plan."""
+        old = self._reqs[i]
+        r = _native.Request()
+        ctypes.pointer(r)[0] = req
+        r.fft_len, r.fft_n1, r.tile_w = old.fft_len, old.fft_n1, old.tile_w
+        r.flags = (int(req.flags) & ~self._PLAN_FLAGS) | (int(old.flags) & self._PLAN_FLAGS)
+        check(self._lib.radis_ldm_batch_set_request(self._h, int(i), byref(r)))
+        self._reqs[i] = r
         self.infos[i] = self._info_nosync(i)
 
     # -- execution --------------------------------------------------------------

@@ -9,6 +9,10 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
+    # the autotuner's cache of this run: tests never read or write the user's
+    import tempfile
+
+    os.environ["RADIS_AMD_TUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="radis_amd_tune_"), "autotune.json")
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X) and the built library")
     config.addinivalue_line("markers", "slow: takes more than a few seconds on the CPU")
 

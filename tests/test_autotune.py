@@ -56,7 +56,7 @@ def test_tuned_schedule_applies_and_matches_oracle(tmp_path, monkeypatch):
     inf = b.info(0)
     assert int(inf.fft_len) == entry["fft_len"] and int(inf.tile_w) == entry["tile_w"]
     ref = oracle.eq_abscoeff(lines.as_dict(), c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction,
-                             c.dxG, c.dxL, L=int(inf.fft_len))
+                             c.dxG, c.dxL)
     assert oracle.rel_peak_error(out, ref) <= 1e-4
     # an explicit schedule is never overridden by the cache
     r2 = engine.make_request(db, c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar, c.mole_fraction, c.dxG, c.dxL,

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _native.load().radis_ldm_abi_version() == _native.ABI_VERSION == 4
+    assert _native.load().radis_ldm_abi_version() == _native.ABI_VERSION == 5
 
 
 @pytest.mark.parametrize("wmin,wmax,wstep", [(1900.0, 2300.0, 0.002), (2000.0, 2500.0, 0.001),
@@ -50,9 +50,15 @@ def test_plan_grid_matches_numpy(wmin, wmax, wstep):
         while x % p == 0:
             x //= p
     assert x == 1
-    # 7-smooth lengths are dense (within 6 % of 2 N_v); a split with
-    # compile-time-sized passes is taken up to 1.35x the smallest of them
-    assert L < 2 * nv * 1.06 * 1.35
+    # the smallest admissible run-time-sized length: 2 N_v itself, or at
+    # least the image correction's floor; a split with compile-time-sized
+    # passes is taken up to 1.35x that length
+    from radis_amd import autotune
+
+    gen = min(Lc for Lc, _, g in autotune.fft_candidates(nv) if g)
+    need = 2 * nv
+    assert L == need or L >= autotune.wrap_floor(need)
+    assert gen <= L <= 1.35 * gen
 
 
 def test_random_windows_grid_lengths():

@@ -62,7 +62,7 @@ def test_derived_match_oracle_compiled_and_generic(dev):
         _check_epilogue(c, res[i], d, L_path)
         inf = b.info(i)
         ref_a = oracle.eq_abscoeff(lines.as_dict(), c.wmin, c.wmax, c.wstep, c.Tgas, c.pressure_bar,
-                                   c.mole_fraction, c.dxG, c.dxL, L=int(inf.fft_len))
+                                   c.mole_fraction, c.dxG, c.dxL)
         ref = oracle.derived_quantities(oracle.wavenumber_grid(c.wmin, c.wmax, c.wstep), ref_a, L_path, c.Tgas)
         for k in KEYS:
             e = _err(k, d[k], ref[k])
@@ -145,9 +145,8 @@ def test_set_derived_rejects_bad_arguments(dev):
 
 
 # --- facade: radis/test/gpu/test_gpu.py -------------------------------------------
-def _oracle_spectrum(lines, wmin, wmax, wstep, T, p, x, path_length, L):
-    a = oracle.eq_abscoeff(lines.as_dict(), wmin, wmax, wstep, T, p, x, engine.DXG_DEFAULT, engine.DXL_DEFAULT,
-                           L=L)
+def _oracle_spectrum(lines, wmin, wmax, wstep, T, p, x, path_length):
+    a = oracle.eq_abscoeff(lines.as_dict(), wmin, wmax, wstep, T, p, x, engine.DXG_DEFAULT, engine.DXL_DEFAULT)
     d = oracle.derived_quantities(oracle.wavenumber_grid(wmin, wmax, wstep), a, path_length, T)
     return dict(abscoeff=a, absorbance=d["absorbance"], emissivity_noslit=d["emissivity"],
                 transmittance_noslit=d["transmittance"], radiance_noslit=d["radiance"])
@@ -155,13 +154,12 @@ def _oracle_spectrum(lines, wmin, wmax, wstep, T, p, x, path_length, L):
 
 def test_eq_spectrum_gpu_matches_oracle():
     """test_gpu.py:62-112: every quantity of eq_spectrum_gpu against the
-    CPU chain (here the oracle at the same FFT length)."""
+    CPU chain (here the oracle, at RADIS's FFT length 2 N_v)."""
     lines = hitran.line_list(HITRAN)
     sf = factory.SpectrumFactory(wavenum_min=2100.0, wavenum_max=2200.0, wstep=0.001, pressure=0.1,
                                  mole_fraction=0.01, path_length=1, lines=lines)
     s = sf.eq_spectrum_gpu(Tgas=1000.0, name="GPU")
-    L = int(s.gpu_app._batch.info(0).fft_len)
-    ref = _oracle_spectrum(lines, 2100.0, 2200.0, 0.001, 1000.0, 0.1, 0.01, 1.0, L)
+    ref = _oracle_spectrum(lines, 2100.0, 2200.0, 0.001, 1000.0, 0.1, 0.01, 1.0)
     for var in ("abscoeff", "absorbance", "emissivity_noslit", "transmittance_noslit", "radiance_noslit"):
         w, y = s.get(var)
         assert y.shape == w.shape == ref[var].shape
@@ -210,9 +208,8 @@ def test_factory_min_rms_option():
     sf = factory.SpectrumFactory(wavenum_min=2150.0, wavenum_max=2200.0, wstep=0.001, pressure=0.05,
                                  mole_fraction=0.1, optimization="min-RMS", lines=lines)
     s = sf.eq_spectrum_gpu(Tgas=2500.0)
-    L = int(s.gpu_app._batch.info(0).fft_len)
     ref = oracle.eq_abscoeff(lines.as_dict(), 2150.0, 2200.0, 0.001, 2500.0, 0.05, 0.1, engine.DXG_DEFAULT,
-                             engine.DXL_DEFAULT, L=L, optimization="min-RMS")
+                             engine.DXL_DEFAULT, optimization="min-RMS")
     assert oracle.rel_peak_error(s.get("abscoeff")[1], ref) <= REL_TOL
     assert s.conditions["optimization"] == "min-RMS"
     s.exit_gpu()

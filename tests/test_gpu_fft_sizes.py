@@ -1,6 +1,6 @@
 """Every compile-time-sized FFT split n1 x n2 against the run-time-sized
 (generic) passes on the same request, and against the FP64 oracle at the
-same FFT length.  The grid fills the transform (2 N_v just below L) so every
+same FFT length (RADIS_LDM_PERIOD_L: the circular result at L itself).  The grid fills the transform (2 N_v just below L) so every
 bin of both passes carries signal through the lineshape product."""
 import numpy as np
 import pytest
@@ -55,8 +55,9 @@ def test_compiled_splits_match_generic_and_oracle(dev, n1):
         db = engine.LineDatabase(dev, lines)
         cond = dict(wmin=WMIN, wmax=wmax, wstep=WSTEP, Tgas=1200.0, pressure_bar=1.0, mole_fraction=0.2,
                     dxG=0.2, dxL=0.2)
-        rc = engine.make_request(db, **cond, fft_len=L, fft_n1=n1)
-        rg = engine.make_request(db, **cond, fft_len=L, fft_n1=n1, generic_fft=True)
+        # the transform's own period (2 N_v is just below L)
+        rc = engine.make_request(db, **cond, fft_len=L, fft_n1=n1, period_fft_len=True)
+        rg = engine.make_request(db, **cond, fft_len=L, fft_n1=n1, generic_fft=True, period_fft_len=True)
         b = engine.SpectrumBatch(db, [rc, rg])
         out = b.results(b.run())
         ic, ig = b.info(0), b.info(1)

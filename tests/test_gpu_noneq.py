@@ -64,7 +64,7 @@ def test_noneq_matches_oracle(dev, Tvib, Trot):
     b = engine.SpectrumBatch(db, [req])
     res = b.results(b.run())[0]
     inf = b.info(0)
-    ref, det = oracle.eq_abscoeff(lines, wmin, wmax, wstep, Trot, p, x, L=int(inf.fft_len), return_details=True,
+    ref, det = oracle.eq_abscoeff(lines, wmin, wmax, wstep, Trot, p, x, return_details=True,
                                   Tvib=Tvib, Trot=Trot)
     t = b.line_terms(0)
     np.testing.assert_array_equal(t["k0"], det["ki0"])

@@ -395,3 +395,45 @@ def test_compressed_par_reads_like_plain(tmp_path, suffix):
     assert a.keys() == b.keys()
     for k in a:
         np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_number_density_is_loschmidts_constant():
+    """Ideal-gas number density of lbl/factory.py:908 at 273.15 K and
+    1.01325 bar (x = 1): Loschmidt's constant, 2.686780111e19 cm-3 (CODATA
+    2018).  A bar/Pa or bar/atm slip is off by 1e5 or 1.3 %."""
+    assert o.number_density(1.0, 1.01325, 273.15) == pytest.approx(2.686780111e19, rel=1e-9)
+    # linear in x and p, inverse in T
+    assert o.number_density(0.1, 2 * 1.01325, 2 * 273.15) == pytest.approx(2.686780111e18, rel=1e-9)
+
+
+def test_lineshift_is_delta_at_one_atmosphere_and_linear_in_pressure():
+    """calc_lineshift (lbl/base.py:2351-2360) shifts by delta_air * p_atm:
+    at p = 1.01325 bar the shift is delta exactly; 10 bar give
+    10/1.01325 delta (a missing bar->atm conversion gives 1.01325 delta at
+    one atmosphere and fails both)."""
+    nu0 = np.array([2000.123456789, 2150.856008, 2299.5])
+    d = np.array([-0.0024, 0.0031, 0.0])
+    np.testing.assert_array_equal(o.lineshift(nu0, d, 1.01325) - nu0, (nu0 + d) - nu0)
+    np.testing.assert_allclose(o.lineshift(nu0, d, 10.0) - nu0, d * (10.0 / 1.01325), rtol=1e-9, atol=1e-15)
+    s1 = o.lineshift(nu0, d, 3.0) - nu0
+    s2 = o.lineshift(nu0, d, 6.0) - nu0
+    np.testing.assert_allclose(s2, 2 * s1, rtol=1e-9, atol=1e-15)
+    np.testing.assert_array_equal(o.lineshift(nu0, None, 5.0), nu0)
+
+
+def test_select_window_keeps_both_ends_and_drops_one_ulp_outside():
+    """Line selection wmin <= nu0 <= wmax (lbl/loader.py:2707): lines exactly
+    on both ends are kept, lines one ulp outside are dropped; the index
+    range equals the boolean mask of a brute-force scan."""
+    wmin, wmax = 2000.0, 2100.0
+    nu0 = np.array([np.nextafter(wmin, -np.inf), wmin, wmin, np.nextafter(wmin, np.inf), 2050.0,
+                    np.nextafter(wmax, -np.inf), wmax, wmax, np.nextafter(wmax, np.inf), 2200.0])
+    i0, i1 = o.select_window(nu0, wmin, wmax)
+    assert (i0, i1) == (1, 8)
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        nu = np.sort(np.round(rng.uniform(1990.0, 2110.0, 200), 1))
+        a, b = sorted(np.round(rng.uniform(1995.0, 2105.0, 2), 1))
+        i0, i1 = o.select_window(nu, a, b)
+        mask = np.array([a <= v <= b for v in nu])
+        assert np.array_equal(np.flatnonzero(mask), np.arange(i0, i1))
