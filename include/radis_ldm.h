@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RADIS_LDM_ABI_VERSION 5
+#define RADIS_LDM_ABI_VERSION 6
 
 enum {
     RADIS_LDM_OK = 0,
@@ -263,6 +263,39 @@ int radis_ldm_batch_line_terms(radis_ldm_batch *b, int i, int64_t *k0, int32_t *
  * row-major [n_g*n_l][n_v+1], rows r = l*N_L + m); the counterpart of the
  * LDM array of radis/lbl/broadening.py:2340-2346 (transposed). */
 int radis_ldm_batch_copy_ldm(radis_ldm_batch *b, int i, float *dst, void *stream);
+
+/* --- no-allocation entry point --------------------------------------------- */
+/* Device results of one request: LDM axis sizes, device status (0 = ok, bits
+ * as radis_ldm_request_info.status) and the FWHM extremes the axes start
+ * from. */
+typedef struct {
+    int32_t n_g, n_l, status, pad_;
+    double wg_min, wg_max, wl_min, wl_max;
+} radis_ldm_result;
+
+/* Bytes of the per-device scratch array of radis_ldm_run_noalloc. */
+#define RADIS_LDM_NOALLOC_BYTES (256ll << 20)
+
+/* One request run entirely in a fixed per-device scratch array: no device
+ * or host allocation, no host-to-device copy from host memory (the job table
+ * travels as a kernel argument), so it can run under hipGraph capture and in
+ * loops that must not allocate.  It has no counterpart in the reference; it
+ * is the variant of radis_ldm_batch_create + radis_ldm_batch_run for one
+ * request, for callers that cannot allocate.  Everything goes on `stream`
+ * (a hipStream_t, NULL = null stream); nothing is synchronised.
+ *   out_dev     device array of n_v floats (abscoeff times 2**-scale_log2)
+ *   result_dev  NULL, or a device radis_ldm_result the run fills
+ *   info        NULL, or host memory for the planned sizes (scale_log2,
+ *               n_v, fft_len, ...; n_g, n_l and status are in result_dev)
+ * A request whose workspace exceeds RADIS_LDM_NOALLOC_BYTES is refused
+ * (RADIS_LDM_ERANGE, nothing launched).  Every call on one device shares
+ * that device's scratch array: calls whose runs overlap on a device (other
+ * threads, or streams not ordered with each other) race, and a run reads
+ * nothing a previous call left there.  Derived quantities and timing are not
+ * offered. */
+int radis_ldm_run_noalloc(radis_ldm_ctx *ctx, const radis_ldm_db *db, const radis_ldm_request *req,
+                          float *out_dev, radis_ldm_result *result_dev, radis_ldm_request_info *info,
+                          void *stream);
 
 const char *radis_ldm_last_error(void);
 int radis_ldm_abi_version(void);

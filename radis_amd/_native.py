@@ -90,7 +90,8 @@ NONEQ = 8
 SCAN_AXIS = 16
 DEVICE_Q = 32
 PERIOD_L = 64
-ABI_VERSION = 5
+ABI_VERSION = 6
+NOALLOC_BYTES = 256 << 20
 
 
 class RequestInfo(ctypes.Structure):
@@ -114,6 +115,19 @@ class RequestInfo(ctypes.Structure):
         ("wl_max", c_double),
         ("axis_scanned", c_int64),
         ("axis_candidates", c_int64),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("n_g", c_int32),
+        ("n_l", c_int32),
+        ("status", c_int32),
+        ("pad_", c_int32),
+        ("wg_min", c_double),
+        ("wg_max", c_double),
+        ("wl_min", c_double),
+        ("wl_max", c_double),
     ]
 
 
@@ -156,6 +170,8 @@ _PROTOTYPES = {
     "radis_ldm_batch_line_terms": (c_int, [c_void_p, c_int] + [c_void_p] * 7 + [c_void_p]),
     "radis_ldm_batch_copy_ldm": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "radis_ldm_slit_convolve": (c_int, [c_void_p, c_int64, c_void_p, c_int32, c_double, c_void_p, c_void_p]),
+    "radis_ldm_run_noalloc": (c_int, [c_void_p, c_void_p, POINTER(Request), c_void_p, c_void_p,
+                                      POINTER(RequestInfo), c_void_p]),
 }
 
 EXPORTED = tuple(_PROTOTYPES)

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -21,6 +22,7 @@
 #include "ldm_fftc.h"
 #include "ldm_slit.h"
 #include "ldm_wrap.h"
+#include "ldm_setup.h"
 
 namespace {
 
@@ -39,15 +41,15 @@ const int FILL_LDS_MAX = 60 * 1024;      // hard limit (plus ~2.2 KB of static a
 const int MIN_TILE_W = 16, MAX_TILE_W = 4096;
 const int N1_MAX = 1024, N2_MAX = 8192;
 
-thread_local std::string g_err;
+// last error of the calling thread (a fixed buffer: reporting an error
+// allocates nothing)
+thread_local char g_err[1024];
 
 int fail(int code, const char* fmt, ...) {
-  char buf[1024];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
   va_end(ap);
-  g_err = buf;
   return code;
 }
 
@@ -72,6 +74,38 @@ int floor_log2(long long x) {
 
 }  // namespace
 
+// Scratch array of radis_ldm_run_noalloc: job table, axes, work items,
+// twiddle tables and workspace of one request.  Every device the library
+// runs on holds its own copy until the process ends.  In host code the name
+// is a stand-in, not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[RADIS_LDM_NOALLOC_BYTES];
+
+namespace {
+
+const int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[SCRATCH_MAX_DEVICES] = {};
+
+// The scratch array's address on `device` (which must be current), taken
+// once per device.
+int scratch_address(int device, char** out) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+    return fail(RADIS_LDM_EINVAL, "device %d beyond the %d supported", device, SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(RADIS_LDM_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch[device] = (char*)p;
+  }
+  *out = g_scratch[device];
+  return 0;
+}
+
+}  // namespace
+
 // ============================================================================
 // objects
 // ============================================================================
@@ -90,6 +124,9 @@ struct radis_ldm_ctx {
   // hipFree (which can synchronise the device).
   std::multimap<size_t, void*> dev_cache, host_cache;
   size_t dev_cached = 0, host_cached = 0;
+  // this device's copy of the no-allocation entry point's scratch array
+  // (its address from hipGetSymbolAddress, taken once per device)
+  char* scratch = nullptr;
   ~radis_ldm_ctx();
 };
 
@@ -222,6 +259,11 @@ struct Req {
 
 }  // namespace
 
+// One launch of an FFT pass: items of one kernel variant are contiguous;
+// cls = the compiled size (n1 for A and C, n2 for B), 0 = generic (pass B:
+// 0 = generic with small tiles, -1 = generic with large tiles).
+struct LaunchGroup { int cls, first, count; };
+
 struct radis_ldm_batch {
   radis_ldm_ctx* ctx = nullptr;
   radis_ldm_db* db = nullptr;
@@ -246,10 +288,8 @@ struct radis_ldm_batch {
   int4* items_d = nullptr;
   size_t items_bytes = 0;
   int n_axis = 0, n_wrap = 0, n_fill = 0, n_a = 0, n_b = 0, n_c = 0;
-  // launches of the FFT passes: items of one kernel variant are contiguous;
-  // cls = the compiled size (n1 for A and C, n2 for B), 0 = generic (pass B:
-  // 0 = generic with small tiles, -1 = generic with large tiles)
-  struct Group { int cls, first, count; };
+  // launches of the FFT passes (see LaunchGroup)
+  using Group = LaunchGroup;
   std::vector<Group> grp_a, grp_b, grp_c;
   bool tiles_valid = false;  // fill tile bounds computed for the current layout
   size_t fill_lds = 0;
@@ -274,17 +314,22 @@ struct radis_ldm_batch {
 namespace {
 
 // ---- FFT planning ----------------------------------------------------------
-std::vector<int> smooth16_list(int maxv) {
-  // 16*a with a 7-smooth, <= maxv
-  std::vector<int> out;
-  for (int a = 1; 16 * a <= maxv; ++a) {
-    int x = a;
-    for (int p : {2, 3, 5, 7})
-      while (x % p == 0) x /= p;
-    if (x == 1) out.push_back(16 * a);
+// 16*a with a 7-smooth, <= maxv, ascending (fixed storage: planning
+// allocates nothing)
+struct SmoothList {
+  int v[512];
+  int n = 0;
+  explicit SmoothList(int maxv) {
+    for (int a = 1; 16 * a <= maxv && n < 512; ++a) {
+      int x = a;
+      for (int p : {2, 3, 5, 7})
+        while (x % p == 0) x /= p;
+      if (x == 1) v[n++] = 16 * a;
+    }
   }
-  return out;
-}
+  const int* begin() const { return v; }
+  const int* end() const { return v + n; }
+};
 
 bool radices_of(int n, int* r, int& cnt) {
   cnt = 0;
@@ -321,8 +366,8 @@ int64_t wrap_floor(int64_t need) {
 // Search L = n1*n2 >= 2*N_v (or == want) with 16 | n1 <= 1024 and
 // 16 | n2 <= n2_max, 7-smooth cofactors; smallest L, then n1 closest to 512.
 int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int only_n1 = 0) {
-  static const std::vector<int> c1 = smooth16_list(N1_MAX);
-  static const std::vector<int> c2 = smooth16_list(N2_MAX);
+  static const SmoothList c1(N1_MAX);
+  static const SmoothList c2(N2_MAX);
   int64_t bestL = -1;
   double bestScore = 1e300;
   for (int n1 : c1) {
@@ -733,6 +778,48 @@ int layout_request(Req& R, size_t& total) {
   return 0;
 }
 
+// Work-item plan of one request (after layout_request): axis items (edge
+// chunks scanned line by line, whole blocks through their width candidates
+// in groups; none when the window is a single-isotopologue list's every
+// line, whose Pareto sets k_finalize evaluates itself), image-correction
+// node pairs, fill tiles and the FFT passes' blocks.  Sets R.sky_whole and
+// R.n_axp (the axis partials).
+LdmItemPlan item_plan(const radis_ldm_db& db, Req& R) {
+  LdmItemPlan P{};
+  P.i0 = R.i0;
+  const bool scan_all = (R.r.flags & RADIS_LDM_SCAN_AXIS) != 0;
+  R.sky_whole = db.has_sky && !db.has_iso && R.i0 == 0 && R.i1 == db.n && R.i1 > R.i0 && !scan_all;
+  auto chunks = [](int64_t a, int64_t e) { return e > a ? (int)((e - a + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK) : 0; };
+  const int64_t fb0 = (R.i0 + LDM_SKY_BLOCK - 1) / LDM_SKY_BLOCK, fb1 = R.i1 / LDM_SKY_BLOCK;
+  if (R.sky_whole) {
+  } else if (db.has_sky && fb1 > fb0 && !scan_all) {
+    P.lo_a = R.i0;
+    P.lo_e = fb0 * LDM_SKY_BLOCK;
+    P.n_lo = chunks(P.lo_a, P.lo_e);
+    // at least LDM_SKY_GROUP_MIN blocks per workgroup (each partial costs
+    // the reduction a read), and at most about LDM_SKY_WGS workgroups
+    P.G = std::max<int64_t>(LDM_SKY_GROUP_MIN, (fb1 - fb0 + LDM_SKY_WGS - 1) / LDM_SKY_WGS);
+    P.g0 = fb0;
+    P.fb1 = fb1;
+    P.n_g = (int)((fb1 - fb0 + P.G - 1) / P.G);
+    P.hi_a = fb1 * LDM_SKY_BLOCK;
+    P.hi_e = R.i1;
+    P.n_hi = chunks(P.hi_a, P.hi_e);
+  } else {
+    P.lo_a = R.i0;
+    P.lo_e = R.i1;
+    P.n_lo = chunks(R.i0, R.i1);
+  }
+  R.n_axp = P.n_lo + P.n_g + P.n_hi;
+  P.n_wrap = (R.wrap_rows + 1) / 2;
+  P.n_fill = R.n_tiles;
+  P.pcap = (R.m_cap + 1) / 2;
+  P.n_colA = R.plan.n2 >> R.bA;
+  P.n_rowB = R.plan.n1 >> R.bB;
+  P.n_colC = R.plan.n2 >> R.bC;
+  return P;
+}
+
 int build_layout(radis_ldm_batch* b) {
   free_layout(b);
   b->jobs_last.clear();
@@ -769,32 +856,19 @@ int build_layout(radis_ldm_batch* b) {
   if (!b->twmem) return fail(RADIS_LDM_ENOMEM, "device allocation of %zu twiddle bytes failed", twsz);
   b->tw_bytes = twsz;
   b->ctx->in_use += (int64_t)twsz;
-  {
-    std::vector<float2> h(twsz / 8);
-    const double TWO_PI = 6.283185307179586476925286766559;
-    for (auto& kv : sub_off) {
-      float2* t = h.data() + kv.second / 8;
-      for (int e = 0; e < kv.first; ++e) {
-        const double a = -TWO_PI * (double)e / (double)kv.first;
-        t[e] = make_float2((float)std::cos(a), (float)std::sin(a));
-      }
-    }
-    for (auto& kv : l_off) {
-      const int64_t L = kv.first;
-      const size_t nhi = (size_t)((L + 1023) >> LDM_TW_LO_BITS);
-      float2* hi = h.data() + kv.second / 8;
-      float2* lo = hi + nhi;
-      for (size_t q = 0; q < nhi; ++q) {
-        const double a = -TWO_PI * (double)((int64_t)(q << LDM_TW_LO_BITS) % L) / (double)L;
-        hi[q] = make_float2((float)std::cos(a), (float)std::sin(a));
-      }
-      for (int e = 0; e < 1024; ++e) {
-        const double a = -TWO_PI * (double)e / (double)L;
-        lo[e] = make_float2((float)std::cos(a), (float)std::sin(a));
-      }
-    }
-    HIPCHK(hipMemcpy(b->twmem, h.data(), twsz, hipMemcpyHostToDevice));
+  // tables computed on the device (k_twiddles; the no-allocation entry
+  // point builds its tables the same way)
+  for (auto& kv : sub_off)
+    hipLaunchKernelGGL(k_twiddles, dim3((kv.first + 255) / 256), dim3(256), 0, nullptr,
+                       (float2*)((char*)b->twmem + kv.second), kv.first, (float2*)nullptr, 0, (float2*)nullptr, 0ll);
+  for (auto& kv : l_off) {
+    const long long nt = ((kv.first + 1023) >> LDM_TW_LO_BITS) + 1024;
+    hipLaunchKernelGGL(k_twiddles, dim3((unsigned)std::min<long long>((nt + 255) / 256, 4096)), dim3(256), 0, nullptr,
+                       (float2*)nullptr, 0, (float2*)nullptr, 0, (float2*)((char*)b->twmem + kv.second),
+                       (long long)kv.first);
   }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(nullptr));
   for (Req& R : b->reqs) {
     char* base = (char*)b->twmem;
     R.tw1 = (const float2*)(base + sub_off[R.plan.n1]);
@@ -802,50 +876,30 @@ int build_layout(radis_ldm_batch* b) {
     R.twl_hi = (const float2*)(base + l_off[R.plan.L]);
     R.twl_lo = R.twl_hi + (size_t)((R.plan.L + 1023) >> LDM_TW_LO_BITS);
   }
-  // --- work items (FFT passes grouped by kernel variant)
+  // --- work items (FFT passes grouped by kernel variant), per request in
+  // the order plan_item gives them
   std::vector<int4> ax, fi, wr;
   std::map<int, std::vector<int4>> ma, mb, mc;
   for (int j = 0; j < b->n; ++j) {
-    const Req& R = b->reqs[j];
-    const int64_t nl = R.i1 - R.i0;
-    {
-      // axis items: lines of partial blocks scanned, whole blocks through
-      // their width candidates (LDM_AX_CHUNK == LDM_SKY_BLOCK); none when
-      // the window is a single-isotopologue list's every line (k_finalize
-      // evaluates the list's Pareto sets instead)
-      int np = 0;
-      b->reqs[j].sky_whole = b->db->has_sky && !b->db->has_iso && R.i0 == 0 && R.i1 == b->db->n && R.i1 > R.i0 &&
-                             !(R.r.flags & RADIS_LDM_SCAN_AXIS);
-      auto scan = [&](int64_t a, int64_t e) {
-        for (int64_t c = a; c < e; c += LDM_AX_CHUNK)
-          ax.push_back(make_int4(j, np++, (int)(c - R.i0), (int)std::min<int64_t>(LDM_AX_CHUNK, e - c)));
-      };
-      const int64_t fb0 = (R.i0 + LDM_SKY_BLOCK - 1) / LDM_SKY_BLOCK, fb1 = R.i1 / LDM_SKY_BLOCK;
-      if (R.sky_whole) {
-      } else if (b->db->has_sky && fb1 > fb0 && !(R.r.flags & RADIS_LDM_SCAN_AXIS)) {
-        scan(R.i0, fb0 * LDM_SKY_BLOCK);
-        // at least LDM_SKY_GROUP_MIN blocks per workgroup (each partial costs
-        // the reduction a read), and at most about LDM_SKY_WGS workgroups
-        const int64_t G = std::max<int64_t>(LDM_SKY_GROUP_MIN, (fb1 - fb0 + LDM_SKY_WGS - 1) / LDM_SKY_WGS);
-        for (int64_t g = fb0; g < fb1; g += G)
-          ax.push_back(make_int4(j, np++, (int)g, -(int)std::min<int64_t>(G, fb1 - g)));
-        scan(fb1 * LDM_SKY_BLOCK, R.i1);
-      } else {
-        scan(R.i0, R.i1);
-      }
-      b->reqs[j].n_axp = np;
-    }
-    (void)nl;
-    for (int m = 0; m < R.wrap_rows; m += 2) wr.push_back(make_int4(j, m / 2, 0, 0));
-    for (int t = 0; t < R.n_tiles; ++t) fi.push_back(make_int4(j, t, 0, 0));
-    const int pcap = (R.m_cap + 1) / 2;
-    const int nA = R.plan.n2 >> R.bA, nC = R.plan.n2 >> R.bC;
+    Req& R = b->reqs[j];
+    const LdmItemPlan P = item_plan(*b->db, R);
+    const int nax = P.n_lo + P.n_g + P.n_hi, bw = nax, bf = bw + P.n_wrap, ba = bf + P.n_fill;
+    const int bb = ba + P.pcap * P.n_colA, bc = bb + P.n_rowB, be = bc + P.n_colC;
     const int ca_ = R.plan.fc ? R.plan.n1 : 0;
     const int cb_ = R.plan.fc ? R.plan.n2 : (R.plan.n2 <= LDM_CAP_B ? 0 : -1);
-    for (int p = 0; p < pcap; ++p)
-      for (int cb = 0; cb < nA; ++cb) ma[ca_].push_back(make_int4(j, p, cb, 0));
-    for (int kb = 0; kb < (R.plan.n1 >> R.bB); ++kb) mb[cb_].push_back(make_int4(j, kb, 0, 0));
-    for (int cb = 0; cb < nC; ++cb) mc[ca_].push_back(make_int4(j, 0, cb, 0));
+    auto put = [&](std::vector<int4>& v, int a, int e) {
+      for (int i = a; i < e; ++i) {
+        int4 t = plan_item(P, i);
+        t.x = j;
+        v.push_back(t);
+      }
+    };
+    put(ax, 0, nax);
+    put(wr, bw, bf);
+    put(fi, bf, ba);
+    put(ma[ca_], ba, bb);
+    put(mb[cb_], bb, bc);
+    put(mc[ca_], bc, be);
   }
   std::vector<int4> fa, fb, fc;
   auto flatten = [](std::map<int, std::vector<int4>>& m, std::vector<int4>& v,
@@ -929,7 +983,7 @@ int fetch_results(radis_ldm_batch* b) {
 template <typename K, typename... Args>
 void launch_part(radis_ldm_batch* b, bool timed, int slot, int kid, bool first, bool last, K kernel, dim3 grid,
                  dim3 block, size_t shmem, hipStream_t s, Args... args) {
-  if (timed) {
+  if (timed && b) {
     hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, s, first ? b->st_ev[slot][2 * kid] : nullptr,
                           last ? b->st_ev[slot][2 * kid + 1] : nullptr, 0, args...);
     b->ev_used[slot][kid] = true;
@@ -947,38 +1001,47 @@ void launch(radis_ldm_batch* b, bool timed, int slot, int kid, K kernel, dim3 gr
 // stage of each kernel id: axis, finalize, tile bounds, fill, fft_a, fft_b, fft_c, wrap
 const int KERNEL_STAGE[RADIS_LDM_N_STAGES] = {0, 1, 2, 3, 4, 5, 6, 7};
 
-// The kernel sequence of one run (what a graph captures).
-int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool capturing = false) {
-  const radis_ldm_db& db = *b->db;
-  const int4* items = b->items_d;
-  for (int k = 0; k < radis_ldm_batch::N_KERNELS; ++k) b->ev_used[slot][k] = false;
-  const LdmJob* cj = b->jobs_d;
-  const LdmAxes* ca = b->axes_d;
-  if (b->n_axis) launch(b, timed, slot, 0, k_axis, dim3(b->n_axis), dim3(256), 0, s, b->jobs_d, items, db.dev);
-  launch(b, timed, slot, 1, k_finalize, dim3(b->n), dim3(256), 0, s, b->jobs_d, b->axes_d, b->res_d, db.dev);
-  items += b->n_axis;
-  if (b->n_wrap) launch(b, timed, slot, 7, k_wrap, dim3(b->n_wrap), dim3(256), 0, s, cj, ca, items);
-  items += b->n_wrap;
-  if (b->n_fill) {
-    // Tile line bounds depend on the grid and, through the pressure-shift
-    // margin, on p; without a shift column they are fixed per layout.
-    if (db.has_delta || !b->tiles_valid) {
-      launch(b, timed, slot, 2, k_tile_bounds, dim3((b->n_fill + 3) / 4), dim3(256), 0, s, b->jobs_d,
-             b->items_d + (items - b->items_d), b->n_fill, db.dev);
-      // a captured launch has not run yet: only an eager run validates the bounds
-      if (!capturing) b->tiles_valid = !db.has_delta;
-    }
-    launch(b, timed, slot, 3, k_fill, dim3(b->n_fill), dim3(LDM_FILL_T), b->fill_lds, s, cj, ca, items, db.dev);
+// Device state and work items one launch sequence runs over: a batch's,
+// or the no-allocation entry point's single request in its scratch array.
+struct LaunchSet {
+  LdmJob* jobs;
+  LdmAxes* axes;
+  LdmResult* res;
+  int4* items;  // axis, image-correction, fill, pass A, pass B, pass C items in this order
+  int n_req;
+  int n_axis, n_wrap, n_fill, n_a, n_b, n_c;
+  const LaunchGroup *ga, *gb, *gc;
+  int nga, ngb, ngc;
+  size_t fill_lds;
+  bool tile_bounds;  // run k_tile_bounds (else the items hold valid tile line ranges)
+  int n_cu;
+};
+
+// The kernel sequence of one run (what a graph captures).  Timed runs (b
+// non-null) stamp per-kernel events on the launch stream.
+int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
+  const int4* items = X.items;
+  const LdmJob* cj = X.jobs;
+  const LdmAxes* ca = X.axes;
+  if (X.n_axis) launch(b, timed, slot, 0, k_axis, dim3(X.n_axis), dim3(256), 0, s, X.jobs, items, dbd);
+  launch(b, timed, slot, 1, k_finalize, dim3(X.n_req), dim3(256), 0, s, X.jobs, X.axes, X.res, dbd);
+  items += X.n_axis;
+  if (X.n_wrap) launch(b, timed, slot, 7, k_wrap, dim3(X.n_wrap), dim3(256), 0, s, cj, ca, items);
+  items += X.n_wrap;
+  if (X.n_fill) {
+    if (X.tile_bounds)
+      launch(b, timed, slot, 2, k_tile_bounds, dim3((X.n_fill + 3) / 4), dim3(256), 0, s, X.jobs,
+             X.items + (items - X.items), X.n_fill, dbd);
+    launch(b, timed, slot, 3, k_fill, dim3(X.n_fill), dim3(LDM_FILL_T), X.fill_lds, s, cj, ca, items, dbd);
   }
-  items += b->n_fill;
-  const int ng_a = (int)b->grp_a.size(), ng_b = (int)b->grp_b.size(), ng_c = (int)b->grp_c.size();
-  for (int g = 0; g < ng_a; ++g) {
-    const auto& G = b->grp_a[g];
+  items += X.n_fill;
+  for (int g = 0; g < X.nga; ++g) {
+    const auto& G = X.ga[g];
     const int4* it = items + G.first;
-    const bool f = g == 0, l = g == ng_a - 1;
+    const bool f = g == 0, l = g == X.nga - 1;
     const dim3 grid(G.count);
     // compiled pass A is persistent: one workgroup per CU loops over the column blocks
-    const dim3 grid_p(std::min(G.count, b->ctx->n_cu));
+    const dim3 grid_p(std::min(G.count, X.n_cu));
 #define FC_A(N) \
   case N: launch_part(b, timed, slot, 4, f, l, k_fft_a_c<N>, grid_p, dim3(FC_T_A), 0, s, cj, ca, it, G.count); break;
     switch (G.cls) {
@@ -987,11 +1050,11 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
     }
 #undef FC_A
   }
-  items += b->n_a;
-  for (int g = 0; g < ng_b; ++g) {
-    const auto& G = b->grp_b[g];
+  items += X.n_a;
+  for (int g = 0; g < X.ngb; ++g) {
+    const auto& G = X.gb[g];
     const int4* it = items + G.first;
-    const bool f = g == 0, l = g == ng_b - 1;
+    const bool f = g == 0, l = g == X.ngb - 1;
     const dim3 grid(G.count);
 #define FC_B(N) \
   case N: launch_part(b, timed, slot, 5, f, l, k_fft_b_c<N>, grid, dim3(FC_T_B), 0, s, cj, ca, it); break;
@@ -1006,11 +1069,11 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
     }
 #undef FC_B
   }
-  items += b->n_b;
-  for (int g = 0; g < ng_c; ++g) {
-    const auto& G = b->grp_c[g];
+  items += X.n_b;
+  for (int g = 0; g < X.ngc; ++g) {
+    const auto& G = X.gc[g];
     const int4* it = items + G.first;
-    const bool f = g == 0, l = g == ng_c - 1;
+    const bool f = g == 0, l = g == X.ngc - 1;
     const dim3 grid(G.count);
 #define FC_C(N) \
   case N: launch_part(b, timed, slot, 6, f, l, k_fft_c_c<N>, grid, dim3(FC_T), 0, s, cj, it, G.count); break;
@@ -1022,6 +1085,39 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
   }
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// A batch's launch sequence.
+int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool capturing = false) {
+  const radis_ldm_db& db = *b->db;
+  for (int k = 0; k < radis_ldm_batch::N_KERNELS; ++k) b->ev_used[slot][k] = false;
+  LaunchSet X{};
+  X.jobs = b->jobs_d;
+  X.axes = b->axes_d;
+  X.res = b->res_d;
+  X.items = b->items_d;
+  X.n_req = b->n;
+  X.n_axis = b->n_axis;
+  X.n_wrap = b->n_wrap;
+  X.n_fill = b->n_fill;
+  X.n_a = b->n_a;
+  X.n_b = b->n_b;
+  X.n_c = b->n_c;
+  X.ga = b->grp_a.data();
+  X.nga = (int)b->grp_a.size();
+  X.gb = b->grp_b.data();
+  X.ngb = (int)b->grp_b.size();
+  X.gc = b->grp_c.data();
+  X.ngc = (int)b->grp_c.size();
+  X.fill_lds = b->fill_lds;
+  // Tile line bounds depend on the grid and, through the pressure-shift
+  // margin, on p; without a shift column they are fixed per layout.
+  X.tile_bounds = db.has_delta || !b->tiles_valid;
+  X.n_cu = b->ctx->n_cu;
+  const int rc = launch_set(X, db.dev, b, s, timed, slot);
+  // a captured launch has not run yet: only an eager run validates the bounds
+  if (!rc && X.n_fill && X.tile_bounds && !capturing) b->tiles_valid = !db.has_delta;
+  return rc;
 }
 
 int enqueue(radis_ldm_batch* b, hipStream_t s, bool timed, int slot) {
@@ -1072,7 +1168,7 @@ radis_ldm_ctx::~radis_ldm_ctx() {
 // ============================================================================
 extern "C" {
 
-const char* radis_ldm_last_error(void) { return g_err.c_str(); }
+const char* radis_ldm_last_error(void) { return g_err; }
 int radis_ldm_abi_version(void) { return RADIS_LDM_ABI_VERSION; }
 
 int radis_ldm_ctx_create(int device, radis_ldm_ctx** out) {
@@ -1083,8 +1179,12 @@ int radis_ldm_ctx_create(int device, radis_ldm_ctx** out) {
   HIPCHK(hipSetDevice(device));
   int n_cu = 0;
   HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+  char* scr = nullptr;
+  const int rc = scratch_address(device, &scr);
+  if (rc) return rc;
   auto* c = new radis_ldm_ctx();
   c->device = device;
+  c->scratch = scr;
   if (n_cu > 0) c->n_cu = n_cu;
   *out = c;
   return 0;
@@ -1500,9 +1600,10 @@ int radis_ldm_batch_create(radis_ldm_ctx* ctx, const radis_ldm_db* db, int n, co
     if (!ok) rc = fail(RADIS_LDM_ENOMEM, "batch bookkeeping allocation failed");
   }
   if (rc) {
-    std::string msg = g_err;
+    char msg[sizeof g_err];
+    std::memcpy(msg, g_err, sizeof msg);
     radis_ldm_batch_destroy(b);
-    g_err = msg;
+    std::memcpy(g_err, msg, sizeof msg);
     return rc;
   }
   *out = b;
@@ -1717,6 +1818,100 @@ int radis_ldm_batch_copy_ldm(radis_ldm_batch* b, int i, float* dst, void* stream
   const size_t cols = (size_t)(R.nv + 1);
   HIPCHK(hipMemcpy2DAsync(dst, cols * 4, (char*)b->work + R.off_ldm, (size_t)R.stride * 4, cols * 4, rows,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+static_assert(sizeof(radis_ldm_result) == sizeof(LdmResult), "radis_ldm_result mirrors LdmResult");
+static_assert(sizeof(LdmJob) <= 2048, "the job table travels as a kernel argument");
+
+int radis_ldm_run_noalloc(radis_ldm_ctx* ctx, const radis_ldm_db* db, const radis_ldm_request* req, float* out_dev,
+                          radis_ldm_result* result_dev, radis_ldm_request_info* info, void* stream) {
+  if (!ctx || !db || !req || !out_dev) return fail(RADIS_LDM_EINVAL, "null argument");
+  if (db->ctx != ctx) return fail(RADIS_LDM_EINVAL, "line list belongs to another context");
+  if (!ctx->scratch) return fail(RADIS_LDM_EINVAL, "context has no scratch array");
+  HIPCHK(hipSetDevice(ctx->device));
+  // plan (host, nothing allocated): sizes, workspace layout, work items
+  Req R;
+  int rc = plan_one(*db, *req, R);
+  if (rc) return rc;
+  R.m_cap = R.m_need;
+  size_t work = 0;
+  rc = layout_request(R, work);
+  if (rc) return rc;
+  const LdmItemPlan P = item_plan(*db, R);
+  const int n_items = plan_items(P);
+  // scratch layout
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    const size_t o = off;
+    off = align_up(off + n, 256);
+    return o;
+  };
+  const size_t o_job = take(sizeof(LdmJob)), o_axes = take(sizeof(LdmAxes)), o_res = take(sizeof(LdmResult));
+  const size_t o_items = take((size_t)n_items * sizeof(int4));
+  const long long nhi = (R.plan.L + 1023) >> LDM_TW_LO_BITS;
+  const size_t o_tw1 = take((size_t)R.plan.n1 * 8), o_tw2 = take((size_t)R.plan.n2 * 8);
+  const size_t o_twl = take((size_t)(nhi + 1024) * 8);
+  const size_t o_work = take(work);
+  if (off > (size_t)RADIS_LDM_NOALLOC_BYTES)
+    return fail(RADIS_LDM_ERANGE, "request needs %zu bytes of scratch, above the %lld of radis_ldm_run_noalloc "
+                                  "(use a batch)", off, (long long)RADIS_LDM_NOALLOC_BYTES);
+  char* const base = ctx->scratch;
+  float2* tw1 = (float2*)(base + o_tw1);
+  float2* tw2 = (float2*)(base + o_tw2);
+  float2* twl = (float2*)(base + o_twl);
+  R.tw1 = tw1;
+  R.tw2 = tw2;
+  R.twl_hi = twl;
+  R.twl_lo = twl + nhi;
+  LdmJob J;
+  fill_job(*db, R, base + o_work, out_dev, J);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_put_job, dim3(1), dim3(64), 0, s, J, (LdmJob*)(base + o_job));
+  const long long n_tw = R.plan.n1 + R.plan.n2 + nhi + 1024;
+  hipLaunchKernelGGL(k_twiddles, dim3((unsigned)std::min<long long>((n_tw + 255) / 256, 4096)), dim3(256), 0, s, tw1,
+                     R.plan.n1, tw2, R.plan.n2, twl, (long long)R.plan.L);
+  hipLaunchKernelGGL(k_make_items, dim3((n_items + 255) / 256), dim3(256), 0, s, (int4*)(base + o_items), P, n_items);
+  HIPCHK(hipGetLastError());
+  const int cls_ac = R.plan.fc ? R.plan.n1 : 0;
+  const int cls_b = R.plan.fc ? R.plan.n2 : (R.plan.n2 <= LDM_CAP_B ? 0 : -1);
+  const LaunchGroup ga{cls_ac, 0, P.pcap * P.n_colA}, gb{cls_b, 0, P.n_rowB}, gc{cls_ac, 0, P.n_colC};
+  LaunchSet X{};
+  X.jobs = (LdmJob*)(base + o_job);
+  X.axes = (LdmAxes*)(base + o_axes);
+  X.res = result_dev ? (LdmResult*)result_dev : (LdmResult*)(base + o_res);
+  X.items = (int4*)(base + o_items);
+  X.n_req = 1;
+  X.n_axis = P.n_lo + P.n_g + P.n_hi;
+  X.n_wrap = P.n_wrap;
+  X.n_fill = P.n_fill;
+  X.n_a = ga.count;
+  X.n_b = gb.count;
+  X.n_c = gc.count;
+  X.ga = &ga;
+  X.nga = 1;
+  X.gb = &gb;
+  X.ngb = 1;
+  X.gc = &gc;
+  X.ngc = 1;
+  X.fill_lds = std::max<size_t>(fill_tile_bytes(R.m_cap, R.tile_w), 4);
+  X.tile_bounds = true;
+  X.n_cu = ctx->n_cu;
+  rc = launch_set(X, db->dev, nullptr, s, false, 0);
+  if (rc) return rc;
+  if (info) {
+    std::memset(info, 0, sizeof *info);
+    info->n_v = R.nv;
+    info->fft_len = R.plan.L;
+    info->n1 = R.plan.n1;
+    info->n2 = R.plan.n2;
+    info->compiled_fft = R.plan.fc ? 1 : 0;
+    info->line_begin = R.i0;
+    info->line_end = R.i1;
+    info->m_cap = R.m_cap;
+    info->tile_w = R.tile_w;
+    info->scale_log2 = R.scale_log2;
+  }
   return 0;
 }
 

@@ -518,6 +518,32 @@ class SpectrumBatch:
             pass
 
 
+def run_noalloc(db: LineDatabase, req, out, result=None, stream=None):
+    """One request through ``radis_ldm_run_noalloc``: the library allocates
+    nothing, its workspace is the device's fixed scratch array.  ``out`` is a
+    float32 device tensor of at least N_v values (abscoeff times
+    2**-scale_log2); ``result``, if given, a device tensor of at least 48
+    bytes that receives the ``radis_ldm_result`` (N_G, N_L, status, width
+    extremes).  Returns the planned RequestInfo (scale_log2, n_v, fft_len).
+    Nothing is synchronised."""
+    torch = _torch()
+    if out.dtype != torch.float32 or not out.is_cuda or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 device tensor")
+    if result is not None and (not result.is_cuda or result.numel() * result.element_size() < 48):
+        raise ValueError("result must be a device tensor of at least 48 bytes")
+    inf = _native.RequestInfo()
+    check(db._lib.radis_ldm_run_noalloc(db.device.handle, db.handle, byref(req), c_void_p(out.data_ptr()),
+                                        c_void_p(result.data_ptr()) if result is not None else None, byref(inf),
+                                        _stream_handle(stream)))
+    return inf
+
+
+def read_result(result):
+    """The ``radis_ldm_result`` in a device tensor (synchronises)."""
+    raw = result.view(-1).view(dtype=__import__("torch").uint8)[:48].cpu().numpy().tobytes()
+    return _native.Result.from_buffer_copy(raw)
+
+
 def describe_status(st):
     parts = []
     if st & _native.ST_AXIS:

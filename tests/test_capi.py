@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _native.load().radis_ldm_abi_version() == _native.ABI_VERSION == 5
+    assert _native.load().radis_ldm_abi_version() == _native.ABI_VERSION == 6
 
 
 @pytest.mark.parametrize("wmin,wmax,wstep", [(1900.0, 2300.0, 0.002), (2000.0, 2500.0, 0.001),
@@ -91,3 +91,4 @@ def test_null_arguments_rejected():
     assert lib.radis_ldm_db_upload(None, None, None) == _native.EINVAL
     assert lib.radis_ldm_batch_create(None, None, 0, None, None) == _native.EINVAL
     assert lib.radis_ldm_batch_run(None, None, None) == _native.EINVAL
+    assert lib.radis_ldm_run_noalloc(None, None, None, None, None, None, None) == _native.EINVAL
