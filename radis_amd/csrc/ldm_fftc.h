@@ -40,7 +40,6 @@ constexpr bool fc_supported(int n) {
   const int f = fc_odd(n);
   return n % f == 0 && fc_pow2(n / f) && n / f >= 64;
 }
-constexpr int fc_pow2_floor(int n) { return n < 2 ? 1 : 2 * fc_pow2_floor(n >> 1); }
 
 // padded LDS offset of a constant element offset c (valid as an addition to
 // pad_idx(e) when c is a multiple of 32, or when e mod 32 + c mod 32 < 32)
@@ -214,17 +213,8 @@ struct TwProg {
 // the full HBM rate, 64 B ones at about 0.6 of it and 32 B ones at about
 // 0.35, so pass A takes the widest tile that fits LDS (one 1024-thread
 // workgroup per CU).
-#define FC_CAP_A 16384 // complex elements of a pass A tile (at most)
-#define FC_T_A 1024
-#define FC_CAP 8192    // complex elements of a pass C tile (at most)
-#define FC_T 512
-#define FC_CAP_B 2048  // complex elements of a pass B tile
-#define FC_T_B 256
-// columns per workgroup of pass A and of pass C for N1, and rows per
-// workgroup of pass B for N2
-constexpr int fc_cols_a(int n1) { return fc_pow2_floor(FC_CAP_A / n1); }
-constexpr int fc_cols_c(int n1) { return fc_pow2_floor(FC_CAP / n1); }
-constexpr int fc_rows_b(int n2) { return FC_CAP_B / n2; }
+// (tile sizes FC_CAP_A, FC_CAP, FC_CAP_B and fc_cols_a / fc_cols_c /
+// fc_rows_b are in ldm_job.h: the host planner sizes the work items with them)
 
 template <int N1>
 __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
@@ -414,18 +404,29 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   // bins below the circular-image cutoff (kk < wrap_k <= N1, k_wrap): with
   // k = k1 + N1 k2 only k2 = 0 (kk = k1) and k2 = N2 - 1 (kk = N1 - k1)
   // qualify.  The first is slot (0, 0) of thread b < B (t >> LB == 0); the
-  // second is slot (s, RL - 1) of the thread with t >> LB == NSL - 1.
-  // w_lo / w_hi hold those bins (-1: none) and w_s the second one's s.
+  // second is slot (s, RL - 1) of the thread with t >> LB == NSL - 1.  The
+  // workgroup's table values (both bins of its B rows, every L-axis node)
+  // are staged in LDS here, so the pair loop reads no global memory for
+  // them (a global load there would wait behind the next pair's prefetch).
+  // w_lo / w_hi hold the row b of those bins (-1: none), w_s the second's s.
+  __shared__ float s_wh[2][B][LDM_MAX_AX];
   const int WK = J.wrap_k;
   int w_lo = -1, w_hi = -1, w_s = 0;
   if (WK) {
-    if ((int)threadIdx.x < B && k1_0 + (int)threadIdx.x < WK) w_lo = k1_0 + threadIdx.x;
+    const int NLw = axes[it.x].nl;
+    for (int i = threadIdx.x; i < 2 * B * NLw; i += T) {
+      const int c = i / (B * NLw), b = (i / NLw) % B, m = i % NLw;
+      const int k1 = k1_0 + b, kk = c ? N1 - k1 : k1;
+      s_wh[c][b][m] = kk < WK ? J.wrap_h[m * WK + kk] : 0.0f;
+    }
+    if ((int)threadIdx.x < B && k1_0 + (int)threadIdx.x < WK) w_lo = threadIdx.x;
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s) {
       const int t = threadIdx.x + s * T;
-      const int kk = N1 - (k1_0 + (t & (B - 1)));
-      if ((t >> LB) == NSL - 1 && kk < WK) { w_hi = kk; w_s = s; }
+      const int b = t & (B - 1);
+      if ((t >> LB) == NSL - 1 && N1 - (k1_0 + b) < WK) { w_hi = b; w_s = s; }
     }
+    __syncthreads();  // WK is the job's: uniform over the workgroup
   }
   float gcache[SH::ITL][SH::RL];
   int g_l = -1;
@@ -464,10 +465,10 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
           }
     }
     if (w_lo >= 0 || w_hi >= 0) {
-      const float* Hm = J.wrap_h + (r % NLr) * WK;
-      if (w_lo >= 0) F[0][0] += Hm[w_lo];
+      const int m = r % NLr;
+      if (w_lo >= 0) F[0][0] += s_wh[0][w_lo][m];
       if (w_hi >= 0) {
-        const float hv = Hm[w_hi];
+        const float hv = s_wh[1][w_hi][m];
 #pragma unroll
         for (int s = 0; s < SH::ITL; ++s)
           if (s == w_s) F[s][SH::RL - 1] += hv;

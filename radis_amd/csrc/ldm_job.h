@@ -43,6 +43,20 @@ static_assert(LDM_AX_CHUNK == LDM_SKY_BLOCK, "a partial block is one scan chunk"
 #define LDM_WRAP_KMAX 256
 #define LDM_WRAP_LC 2048
 
+// Compile-time-sized FFT passes (ldm_fftc.h): tile sizes and threads.
+#define FC_CAP_A 16384 // complex elements of a pass A tile (at most)
+#define FC_T_A 1024
+#define FC_CAP 8192    // complex elements of a pass C tile (at most)
+#define FC_T 512
+#define FC_CAP_B 2048  // complex elements of a pass B tile
+#define FC_T_B 256
+constexpr int fc_pow2_floor(int n) { return n < 2 ? 1 : 2 * fc_pow2_floor(n >> 1); }
+// columns per workgroup of pass A and of pass C for N1, and rows per
+// workgroup of pass B for N2
+constexpr int fc_cols_a(int n1) { return fc_pow2_floor(FC_CAP_A / n1); }
+constexpr int fc_cols_c(int n1) { return fc_pow2_floor(FC_CAP / n1); }
+constexpr int fc_rows_b(int n2) { return FC_CAP_B / n2; }
+
 // device status codes (LdmJob::status)
 #define LDM_ST_OK 0
 #define LDM_ST_AXIS 1      // N_G or N_L above LDM_MAX_AX
@@ -60,6 +74,43 @@ struct LdmQSpline {
   int n, k;
   double lo, hi;
 };
+
+// Q(T) from a partition-function B-spline, evaluated as FITPACK's splev
+// does (scipy.interpolate.splev with ext=0; the reference interpolates its
+// tabulated Q(T) with splrep/splev, radis/levels/partfunc_cdsd.py:84-117):
+// the knot interval l is the largest l in [k, n-k-2] with t[l] <= x (so the
+// end pieces extrapolate), the k+1 B-splines that are nonzero there come
+// from fpbspl's Cox-de Boor recursion, and Q = sum_j c[l-k+j] B_j.  Shared
+// by the device (k_finalize, k_qspline_eval) and the host (scale bound).
+__host__ __device__ inline double qspline_eval(const LdmQSpline& S, double x) {
+#pragma clang fp contract(off)
+  const int k = S.k;
+  int l = k, r = S.n - k - 2;
+  while (l < r) {
+    const int mid = (l + r + 1) >> 1;
+    if (S.t[mid] <= x) l = mid;
+    else r = mid - 1;
+  }
+  double h[LDM_QS_MAXDEG + 1], hh[LDM_QS_MAXDEG];
+  h[0] = 1.0;
+  for (int j = 1; j <= k; ++j) {
+    for (int i = 0; i < j; ++i) hh[i] = h[i];
+    h[0] = 0.0;
+    for (int i = 1; i <= j; ++i) {
+      const int li = l + i, lj = li - j;
+      if (S.t[li] == S.t[lj]) {
+        h[i] = 0.0;
+        continue;
+      }
+      const double f = hh[i - 1] / (S.t[li] - S.t[lj]);
+      h[i - 1] = h[i - 1] + f * (S.t[li] - x);
+      h[i] = f * (x - S.t[lj]);
+    }
+  }
+  double q = 0.0;
+  for (int j = 0; j <= k; ++j) q = q + S.c[l - k + j] * h[j];
+  return q;
+}
 
 struct LdmDb {
   const double* nu0;
@@ -148,6 +199,7 @@ struct LdmJob {
   const float2* tw2;             // exp(-2 pi i e/n2), e < n2
   const float2* twl_hi;          // W_L^(h*1024)
   const float2* twl_lo;          // W_L^(e), e < 1024
+  const float2* tw_wrap;         // exp(-2 pi i e/LDM_WRAP_LC), e < LDM_WRAP_LC (k_wrap's transform)
   unsigned long long* ax_part;   // [n_ax_part][4]: per-item width extremes (wl min, wl max, wg min, wg max) as bits
   float* wrap_h;                 // [wrap_rows][wrap_k]: image correction of the lineshape FT at bins kk < wrap_k, per L-axis node
   int wrap_k;                    // 0: the transform length is the period (L = 2 N_v, or the request asked for L's own period)

@@ -296,42 +296,6 @@ __global__ __launch_bounds__(256) void k_axis(LdmJob* __restrict__ jobs, const i
   }
 }
 
-// Q(T) from a partition-function B-spline, evaluated as FITPACK's splev
-// does (scipy.interpolate.splev with ext=0; the reference interpolates its
-// tabulated Q(T) with splrep/splev, radis/levels/partfunc_cdsd.py:84-117):
-// the knot interval l is the largest l in [k, n-k-2] with t[l] <= x (so the
-// end pieces extrapolate), the k+1 B-splines that are nonzero there come
-// from fpbspl's Cox-de Boor recursion, and Q = sum_j c[l-k+j] B_j.  Shared
-// by the device (k_finalize, k_qspline_eval) and the host (scale bound).
-__host__ __device__ inline double qspline_eval(const LdmQSpline& S, double x) {
-#pragma clang fp contract(off)
-  const int k = S.k;
-  int l = k, r = S.n - k - 2;
-  while (l < r) {
-    const int mid = (l + r + 1) >> 1;
-    if (S.t[mid] <= x) l = mid;
-    else r = mid - 1;
-  }
-  double h[LDM_QS_MAXDEG + 1], hh[LDM_QS_MAXDEG];
-  h[0] = 1.0;
-  for (int j = 1; j <= k; ++j) {
-    for (int i = 0; i < j; ++i) hh[i] = h[i];
-    h[0] = 0.0;
-    for (int i = 1; i <= j; ++i) {
-      const int li = l + i, lj = li - j;
-      if (S.t[li] == S.t[lj]) {
-        h[i] = 0.0;
-        continue;
-      }
-      const double f = hh[i - 1] / (S.t[li] - S.t[lj]);
-      h[i - 1] = h[i - 1] + f * (S.t[li] - x);
-      h[i] = f * (x - S.t[lj]);
-    }
-  }
-  double q = 0.0;
-  for (int j = 0; j <= k; ++j) q = q + S.c[l - k + j] * h[j];
-  return q;
-}
 
 // Q(T) of one slot at n temperatures (radis_ldm_qspline_eval)
 __global__ __launch_bounds__(256) void k_qspline_eval(const LdmQSpline* __restrict__ qs, int iso,

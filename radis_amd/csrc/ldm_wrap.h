@@ -19,11 +19,14 @@
 // sigma = w / LDM_WRAP_NSIG.  c_ext is even, so its DFT H_m is real, and it
 // is negligible beyond K = wrap_k bins: pass B adds H_m[kk] to the
 // lineshape FT of every row on node m at its bins kk < K.  H_m comes from
-// LDM_WRAP_LC samples of c_ext (the sampled sum equals the full one up to
-// aliases of H_m beyond LDM_WRAP_LC - K, far below 1e-9 of it).
+// LC >= 8 K samples of c_ext (512, 1024 or LDM_WRAP_LC; the sampled sum
+// equals the full one up to aliases of H_m beyond LC - K, far below 1e-9 of
+// it).
 //
-// One workgroup per (request, pair of L-axis nodes): the two real sequences
-// are the real and imaginary parts of one complex transform.
+// One 1024-thread workgroup per (request, pair of L-axis nodes): the two
+// real sequences are the real and imaginary parts of one complex transform
+// (one or two samples per thread; the stage twiddles come from a table
+// computed with the batch's other twiddles).
 #pragma once
 #include "ldm_fftc.h"
 
@@ -41,31 +44,23 @@ __device__ __forceinline__ WrapNode wrap_node(double gamma, double period) {
 }
 
 // P(d) of one period in the overflow-free form (1/L)(1 - E^2) / ((1 - E)^2 +
-// 4 E sin^2(pi d / L)) (sinh and cosh multiplied by 2 e^-a)
-__device__ __forceinline__ double lor_periodic(const WrapNode& w, double d, double inv_period) {
-  const double s = sinpi(d * inv_period);
-  return inv_period * w.omE2 / (w.omE * w.omE + 4.0 * w.E * s * s);
+// 4 E sin^2(pi d / L)) (sinh and cosh multiplied by 2 e^-a); s2 = sin^2(pi d / L)
+__device__ __forceinline__ double lor_periodic(const WrapNode& w, double s2, double inv_period) {
+  return inv_period * w.omE2 / (w.omE * w.omE + 4.0 * w.E * s2);
 }
 
-__global__ __launch_bounds__(256) void k_wrap(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
-                                              const int4* __restrict__ items) {
-  constexpr int LC = LDM_WRAP_LC, T = 256;
-  __shared__ float2 buf[LdsSize<LC>::value];
-  __shared__ float2 s_tw[LC];
-  const int4 it = items[blockIdx.x];
-  const LdmJob& J = jobs[it.x];
-  const LdmAxes& A = axes[it.x];
+#define LDM_WRAP_T 1024
+
+// The table of one node pair from LC samples (LC >= 8 K: 512, 1024 or 2048).
+template <int LC>
+__device__ __forceinline__ void wrap_pair(const LdmJob& J, const LdmAxes& A, int ma, float2* __restrict__ buf,
+                                          float2* __restrict__ s_tw) {
+  constexpr int T = LDM_WRAP_T, STRIDE = LDM_WRAP_LC / LC;
   const int K = J.wrap_k;
-  const int nl = A.nl;
-  const int ma = 2 * it.y, mb = ma + 1;
-  if (J.status || K == 0 || ma >= nl) return;
-  const bool has_b = mb < nl;
-  // forward table W_LC^e
-  for (int e = threadIdx.x; e < LC; e += T) {
-    double sn, cs;
-    sincospi(-2.0 * (double)e / (double)LC, &sn, &cs);
-    s_tw[e] = make_float2((float)cs, (float)sn);
-  }
+  const int mb = ma + 1;
+  const bool has_b = mb < A.nl;
+  // W_LC^e = W_LDM_WRAP_LC^(e * STRIDE)
+  for (int e = threadIdx.x; e < LC; e += T) s_tw[e] = gld(J.tw_wrap, 8u * (unsigned)(e * STRIDE));
   const double Lp = (double)J.L, Ls = (double)J.wrap_ls;
   const double nv = 0.5 * Ls;
   const double w = fmin(Lp - Ls, 0.5 * nv);
@@ -78,13 +73,16 @@ __global__ __launch_bounds__(256) void k_wrap(const LdmJob* __restrict__ jobs, c
   const double ga = exp(A.laL[ma]) * h, gb = has_b ? exp(A.laL[mb]) * h : ga;
   const WrapNode sa = wrap_node(ga, Ls), pa = wrap_node(ga, Lp);
   const WrapNode sb = wrap_node(gb, Ls), pb = wrap_node(gb, Lp);
-  // phi(x) c(x) for both nodes (x in [0, L'))
+  // phi(x) c(x) for both nodes (x in [0, L')): the two nodes share x's
+  // sines; phi needs no more than FP32 (it is 1 - 3e-7 at x = N_v)
   auto side = [&](double x, double& va, double& vb) {
     const double z = (x - d_mid) * isig;
     if (z >= z_cut) return;
-    const double phi = 0.5 * erfc(z);
-    va += phi * (lor_periodic(sa, x, inv_s) - lor_periodic(pa, x, inv_p));
-    if (has_b) vb += phi * (lor_periodic(sb, x, inv_s) - lor_periodic(pb, x, inv_p));
+    const double phi = 0.5 * (double)erfcf((float)z);
+    const double ss = sinpi(x * inv_s), sp = sinpi(x * inv_p);
+    const double s2s = ss * ss, s2p = sp * sp;
+    va += phi * (lor_periodic(sa, s2s, inv_s) - lor_periodic(pa, s2p, inv_p));
+    if (has_b) vb += phi * (lor_periodic(sb, s2s, inv_s) - lor_periodic(pb, s2p, inv_p));
   };
   const double step = Lp / (double)LC;
   for (int j = threadIdx.x; j < LC; j += T) {
@@ -104,4 +102,21 @@ __global__ __launch_bounds__(256) void k_wrap(const LdmJob* __restrict__ jobs, c
     H[ma * K + k] = z.x * sc;
     if (has_b) H[mb * K + k] = z.y * sc;
   }
+}
+
+__global__ __launch_bounds__(LDM_WRAP_T) void k_wrap(const LdmJob* __restrict__ jobs,
+                                                     const LdmAxes* __restrict__ axes,
+                                                     const int4* __restrict__ items) {
+  __shared__ float2 buf[LdsSize<LDM_WRAP_LC>::value];
+  __shared__ float2 s_tw[LDM_WRAP_LC];
+  const int4 it = items[blockIdx.x];
+  const LdmJob& J = jobs[it.x];
+  const LdmAxes& A = axes[it.x];
+  const int K = J.wrap_k;
+  const int ma = 2 * it.y;
+  if (J.status || K == 0 || ma >= A.nl) return;
+  // fewest samples with LC >= 8 K (uniform per workgroup)
+  if (8 * K <= 512) wrap_pair<512>(J, A, ma, buf, s_tw);
+  else if (8 * K <= 1024) wrap_pair<1024>(J, A, ma, buf, s_tw);
+  else wrap_pair<LDM_WRAP_LC>(J, A, ma, buf, s_tw);
 }

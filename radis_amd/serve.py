@@ -74,6 +74,7 @@ class SpectrumServer:
         self.done = []
         self.rejected = []
         self._next = 0
+        self.max_in_flight = 0  # most batches in flight at once
 
     # -- requests ---------------------------------------------------------------
     def submit(self, db_name, cond, t_arrival=None):
@@ -144,6 +145,7 @@ class SpectrumServer:
             r.t_start = t0
         self.flights.append(_Flight(batch, picked, outs, host, ev, stream, nbytes))
         self.in_flight_bytes += nbytes
+        self.max_in_flight = max(self.max_in_flight, len(self.flights))
         return True
 
     def _complete(self):

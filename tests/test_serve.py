@@ -50,6 +50,9 @@ def test_results_same_alone_and_in_the_mix(server_setup):
         arrivals.append((0.002 * i + 0.001, "h2o", b[i]))
     done = srv.run_stream(arrivals, timeout=120)
     assert len(done) == 48 and all(r.error is None for r in done)
+    # continuous batching: batches overlapped on the device
+    print(f"most batches in flight at once: {srv.max_in_flight}")
+    assert srv.max_in_flight >= 2
     for r in done[::5]:
         db = dbs[r.db]
         alone = engine.SpectrumBatch(db, [engine.request_from_conditions(db, r.cond)])
