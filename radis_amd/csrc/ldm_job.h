@@ -151,6 +151,7 @@ struct LdmJob {
   long long i0, i1;              // window line range
   double p_atm, x_self, x_air;   // pressure (atm), mole fractions
   double log_r;                  // log(Tref/T)
+  double ratio_T;                // Tref/T (the base of the FP64 width powers)
   double c_boltz;                // -hc_k*(1/T - 1/Tref)
   double i_scale;                // number density * 2^-scale_log2
   double dopp[LDM_MAX_ISO];      // sqrt((2 Na k_b T ln2)/M_iso)

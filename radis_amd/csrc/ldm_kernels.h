@@ -129,12 +129,13 @@ __device__ __forceinline__ double fwhm_gauss(double nu0, double dopp) {
 }
 
 // Lorentzian FWHM, radis/lbl/broadening.py:341-383 (x2 at 1854), with
-// diluent {'air': 1-x}; (Tref/T)**n evaluated as exp(n*log(Tref/T)).
+// diluent {'air': 1-x}; (Tref/T)**n as pow, as NumPy evaluates it (the
+// width extremes the LDM axes start from).
 __device__ __forceinline__ double fwhm_lorentz(const LdmJob& J, double g_air, double g_self, double n_air,
                                                double n_self) {
 #pragma clang fp contract(off)
-  const double ra = exp(n_air * J.log_r);
-  const double rs = (n_self == n_air) ? ra : exp(n_self * J.log_r);
+  const double ra = pow(J.ratio_T, n_air);
+  const double rs = (n_self == n_air) ? ra : pow(J.ratio_T, n_self);
   const double air = ((ra * g_air) * J.p_atm) * J.x_air;
   const double slf = rs * ((g_self * J.p_atm) * J.x_self);
   return (air + slf) * 2.0;
@@ -721,6 +722,9 @@ __device__ __forceinline__ bool near_node(float p, int n) {
 __device__ __forceinline__ double fwhm_lorentz_c(const LineConst& C, double g_air, double g_self, double n_air,
                                                  double n_self) {
 #pragma clang fp contract(off)
+  // exp(n log(Tref/T)) here (pow, as in fwhm_lorentz, would push the fill's
+  // registers into spills): within 2 ulp of NumPy's pow; it only places
+  // lines within 2e-4 of an axis node
   const double ra = exp(n_air * C.log_r);
   const double rs = (n_self == n_air) ? ra : exp(n_self * C.log_r);
   const double air = ((ra * g_air) * C.p_atm) * C.x_air;

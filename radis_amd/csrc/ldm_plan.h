@@ -657,6 +657,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.x_self = r.mole_fraction;
   J.x_air = 1 - r.mole_fraction;
   J.log_r = std::log(r.Tref / r.Tgas);
+  J.ratio_T = r.Tref / r.Tgas;
   J.c_boltz = -HC_K * (1 / r.Tgas - 1 / r.Tref);
   const double dens = r.mole_fraction * ((r.pressure_bar * 1e5) / (K_KB * r.Tgas)) * 1e-6;  // factory.py:908
   J.i_scale = std::ldexp(dens, -R.scale_log2);
