@@ -864,6 +864,17 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   __syncthreads();
   const int wlen = (int)(k_hi - k_lo);
   const int sg = nl * Wp;
+  // Corner order per lane.  The 8 atomics of a line go to its stencil's
+  // cells (column c0+dc, G row l0+dl, L row m0+dm); lanes whose lines share
+  // a stencil (on a dense list most of a wave: one grid column, one G row,
+  // a dozen L rows) would hit the same cell in the same instruction, and
+  // same-address lanes serialise.  Lane bits (x0, x1, x2) map the
+  // instruction for corner (dc, dl, dm) to the cell (dc^x0, dl^x1, dm^x2),
+  // so such lanes spread over 8 cells.  Each cell still receives exactly
+  // the product it received before (same operands, same order).
+  const int x0 = threadIdx.x & 1, x1 = (threadIdx.x >> 1) & 1, x2 = (threadIdx.x >> 2) & 1;
+  const int s_c = x0 ? -1 : 1, s_g = x1 ? -sg : sg, s_l = x2 ? -Wp : Wp;  // steps from the lane's first cell
+  const int x_off = x0 + (x1 ? sg : 0) + (x2 ? Wp : 0);                  // the lane's first cell
   for (unsigned i = first; i < hi; i += LDM_FILL_T) {
     const LineRaw cur = n1;
     const IsoVals cv = v1;
@@ -875,23 +886,29 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     line_terms<NONEQ>(C, cur, cv, ng, nl, t);
     const int c0 = t.k0 - (int)k_lo;  // column of corner k0 in the tile, in [-1, W-1] if touching
     if (c0 < -1 || c0 > wlen - 1) continue;
-    // the 8 corners are base + {0, 1} + {0, nl*Wp} + {0, Wp}
-    double* base = acc + (t.l0 * nl + t.m0) * Wp + c0;
-    const float w00 = (1.0f - t.tg) * (1.0f - t.tl), w01 = (1.0f - t.tg) * t.tl;
-    const float w10 = t.tg * (1.0f - t.tl), w11 = t.tg * t.tl;
-    if (c0 >= 0) {
-      const float v = t.I * (1.0f - t.tv);
+    // the stencil's cells are base + {0, 1} + {0, nl*Wp} + {0, Wp}; the
+    // weight of cell (c, l, m) is fc[c] * (fg[l] * fl[m])
+    double* base = acc + (t.l0 * nl + t.m0) * Wp + c0 + x_off;
+    const float ug = 1.0f - t.tg, ul = 1.0f - t.tl;
+    const float fg0 = x1 ? t.tg : ug, fg1 = x1 ? ug : t.tg;  // this lane's corner dl = 0, 1
+    const float fl0 = x2 ? t.tl : ul, fl1 = x2 ? ul : t.tl;
+    const float w00 = fg0 * fl0, w01 = fg0 * fl1, w10 = fg1 * fl0, w11 = fg1 * fl1;
+    const float v0 = t.I * (1.0f - t.tv), v1 = t.I * t.tv;
+    // columns c0 (needs c0 >= 0) and c0 + 1 (needs c0 + 1 < wlen)
+    if (x0 ? (c0 + 1 < wlen) : (c0 >= 0)) {
+      const float v = x0 ? v1 : v0;
       atomicAdd(base, (double)(v * w00));
-      atomicAdd(base + Wp, (double)(v * w01));
-      atomicAdd(base + sg, (double)(v * w10));
-      atomicAdd(base + sg + Wp, (double)(v * w11));
+      atomicAdd(base + s_l, (double)(v * w01));
+      atomicAdd(base + s_g, (double)(v * w10));
+      atomicAdd(base + s_g + s_l, (double)(v * w11));
     }
-    if (c0 + 1 < wlen) {
-      const float v = t.I * t.tv;
-      atomicAdd(base + 1, (double)(v * w00));
-      atomicAdd(base + 1 + Wp, (double)(v * w01));
-      atomicAdd(base + 1 + sg, (double)(v * w10));
-      atomicAdd(base + 1 + sg + Wp, (double)(v * w11));
+    if (x0 ? (c0 >= 0) : (c0 + 1 < wlen)) {
+      const float v = x0 ? v0 : v1;
+      double* b1 = base + s_c;
+      atomicAdd(b1, (double)(v * w00));
+      atomicAdd(b1 + s_l, (double)(v * w01));
+      atomicAdd(b1 + s_g, (double)(v * w10));
+      atomicAdd(b1 + s_g + s_l, (double)(v * w11));
     }
   }
   __syncthreads();
