@@ -45,6 +45,18 @@ constexpr bool fc_supported(int n) {
 // pad_idx(e) when c is a multiple of 32, or when e mod 32 + c mod 32 < 32)
 constexpr int pad_c(int c) { return c + ((c >> 5) << 1); }
 
+// LDS paddings: PD = 5 is pad_idx (2 slots after every 32 elements); PD = 4
+// puts 1 slot after every 16, which makes the stride-16 writes of a radix-16
+// stage over one column (B = 1) conflict-free.  Both add N/16 slots.
+template <int PD>
+__device__ __forceinline__ int pad_e(int e) {
+  return PD == 5 ? pad_idx(e) : e + (e >> 4);
+}
+template <int PD>
+constexpr int pad_ce(int c) {
+  return PD == 5 ? pad_c(c) : c + (c >> 4);
+}
+
 // One Stockham DIT stage (see ldm_fft.h for the recursion) with compile-time
 // N, radix R, Ns, B and T.  Element e = i*B + b is element i of column b.
 // IN_REG: the stage's inputs come from ld(s, q, e) instead of LDS (the first
@@ -62,18 +74,25 @@ __device__ __forceinline__ int fc_tid() {
   return t;
 }
 
-template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, class LD, class ST>
+// PD: the LDS padding (pad_e).  TWO >= 0: stage twiddles from a per-stage
+// table (fc_load_tw_stages) at tw[TWO + (q-1)*NS + k], so lanes with
+// consecutive k read consecutive slots; TWO < 0: from the W_N table at
+// tw[q*k*N/(NS*R)], whose power-of-two strides put a wave's lanes on a few
+// LDS banks when B is small.
+template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD, int TWO, class LD,
+          class ST>
 __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st) {
   const int tid = OPQ ? fc_tid() : (int)threadIdx.x;
   constexpr int NR = N / R;
   constexpr int ITEMS = NR * B;
   constexpr int IT = (ITEMS + T - 1) / T;
   constexpr int LB = fc_log2(B);
-  // read offsets q*NR*B pad linearly when NR*B is a multiple of 32
-  constexpr bool RD_LIN = (NR * B) % 32 == 0;
-  // write offsets q*NS*B: linear when NS*B is a multiple of 32, or when NS*B
-  // divides 32 and the group base (j - k)*R*B is a multiple of 32
-  constexpr bool WR_LIN = (NS * B) % 32 == 0 || (32 % (NS * B) == 0 && (NS * R * B) % 32 == 0);
+  constexpr int G = 1 << PD;  // padding group
+  // read offsets q*NR*B pad linearly when NR*B is a multiple of G
+  constexpr bool RD_LIN = (NR * B) % G == 0;
+  // write offsets q*NS*B: linear when NS*B is a multiple of G, or when NS*B
+  // divides G and the group base (j - k)*R*B is a multiple of G
+  constexpr bool WR_LIN = (NS * B) % G == 0 || (G % (NS * B) == 0 && (NS * R * B) % G == 0);
   float2 v[IT][R];
 #pragma unroll
   for (int s = 0; s < IT; ++s) {
@@ -83,12 +102,12 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
 #pragma unroll
         for (int q = 0; q < R; ++q) v[s][q] = ld(s, q, it + q * NR * B);
       } else if constexpr (RD_LIN) {
-        const float2* src = buf + pad_idx(it);
+        const float2* src = buf + pad_e<PD>(it);
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[s][q] = src[pad_c(q * NR * B)];
+        for (int q = 0; q < R; ++q) v[s][q] = src[pad_ce<PD>(q * NR * B)];
       } else {
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[s][q] = buf[pad_idx(it + q * NR * B)];
+        for (int q = 0; q < R; ++q) v[s][q] = buf[pad_e<PD>(it + q * NR * B)];
       }
       if constexpr (NS > 1) {
         const int j = it >> LB;
@@ -96,7 +115,7 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
         const int e = k * (N / (NS * R));
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          const float2 w = tw[q * e];
+          const float2 w = TWO >= 0 ? tw[TWO + (q - 1) * NS + k] : tw[q * e];
           v[s][q] = FWD ? cmul(v[s][q], w) : cmulc(v[s][q], w);
         }
       }
@@ -116,12 +135,12 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
 #pragma unroll
         for (int q = 0; q < R; ++q) st(s, q, e0 + q * NS * B, v[s][q]);
       } else if constexpr (WR_LIN) {
-        float2* dst = buf + pad_idx(e0);
+        float2* dst = buf + pad_e<PD>(e0);
 #pragma unroll
-        for (int q = 0; q < R; ++q) dst[pad_c(q * NS * B)] = v[s][q];
+        for (int q = 0; q < R; ++q) dst[pad_ce<PD>(q * NS * B)] = v[s][q];
       } else {
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[pad_idx(e0 + q * NS * B)] = v[s][q];
+        for (int q = 0; q < R; ++q) buf[pad_e<PD>(e0 + q * NS * B)] = v[s][q];
       }
     }
   }
@@ -139,17 +158,20 @@ struct FcNone {
 // touches LDS only between its stages.
 // OPQ: the stages take the thread index through fc_tid() (for callers that
 // run the transform inside a loop).
-template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int S = 0, class LD, class ST, class AF,
-          class BL>
+// PD: the LDS padding (pad_e).  TWS: tw is the per-stage twiddle table
+// (fc_load_tw_stages) instead of W_N.
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD = 5, bool TWS = false, int S = 0,
+          class LD, class ST, class AF, class BL>
 __device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
                                           AF& after_first, BL& before_last) {
   constexpr int NST = fc_nstages(N);
   if constexpr (S < NST) {
     if constexpr (S == NST - 1) before_last();
-    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ>(buf, tw, ld,
-                                                                                                             st);
+    constexpr int TWO = (TWS && S > 0) ? fc_ns(N, S) - fc_ns(N, 1) : -1;
+    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ, PD, TWO>(
+        buf, tw, ld, st);
     if constexpr (S == 0) after_first();
-    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, OPQ, S + 1>(buf, tw, ld, st, after_first, before_last);
+    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, S + 1>(buf, tw, ld, st, after_first, before_last);
   }
 }
 template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST, class AF>
@@ -180,6 +202,24 @@ template <int N, int T>
 __device__ __forceinline__ void fc_load_tw(float2* __restrict__ s_tw, const float2* __restrict__ tw) {
 #pragma unroll
   for (int e = threadIdx.x; e < N; e += T) s_tw[e] = gld(tw, 8u * (unsigned)e);
+}
+
+// Per-stage twiddle tables from the W_N table tw (global): stage
+// S >= 1 (radix R, NS) holds W_N^(q k N/(NS R)), q in [1, R), k < NS, at
+// fc_ns(N, S) - fc_ns(N, 1) + (q-1)*NS + k.  The stages' tables add up to
+// N - fc_ns(N, 1) < N entries.
+template <int N, int T, int S = 1>
+__device__ __forceinline__ void fc_load_tw_stages(float2* __restrict__ s_tw, const float2* __restrict__ tw) {
+  if constexpr (S < fc_nstages(N)) {
+    constexpr int NS = fc_ns(N, S), R = fc_radix(N, S), LNS = fc_log2(NS);
+    constexpr int O = NS - fc_ns(N, 1), STEP = N / (NS * R);
+    static_assert(fc_pow2(NS), "stages after the first have power-of-two NS");
+    for (int i = threadIdx.x; i < NS * (R - 1); i += T) {
+      const int q = (i >> LNS) + 1, k = i & (NS - 1);
+      s_tw[O + i] = gld(tw, 8u * (unsigned)(q * k * STEP));
+    }
+    fc_load_tw_stages<N, T, S + 1>(s_tw, tw);
+  }
 }
 
 // Twiddles W_L^(m * (j0 + s*DS + q*DQ)) of one thread's last-stage outputs
@@ -346,7 +386,25 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   const int k1_0 = it.y * B;
   const long long L = J.L;
   const int P = axes[it.x].p, M = axes[it.x].m;
-  fc_load_tw<N2, T>(s_tw, J.tw2);
+  fc_load_tw_stages<N2, T>(s_tw, J.tw2);
+  // The rows' lineshape coefficients (a_l, b_m, 1/F(0), fold count) staged
+  // in LDS.  Read from global memory they are scalar loads, and the LDS
+  // waits of the transform stages then also wait for them.
+  constexpr int ROWS_LDS = 512;
+  __shared__ float4 s_row[ROWS_LDS];
+  const bool rows_lds = M <= ROWS_LDS;
+  if (rows_lds) {
+    for (int r = threadIdx.x; r < M; r += T) {
+      const unsigned o = 4u * (unsigned)r;
+      s_row[r] = make_float4(gld(J.row_a, o), gld(J.row_b, o), gld(J.row_inv_f0, o), __int_as_float(gld(J.nfold, o)));
+    }
+  }
+  __syncthreads();
+  auto row_c = [&](int r) -> float4 {
+    if (rows_lds) return s_row[r];
+    const unsigned o = 4u * (unsigned)r;
+    return make_float4(gld(J.row_a, o), gld(J.row_b, o), gld(J.row_inv_f0, o), __int_as_float(gld(J.nfold, o)));
+  };
   float2 acc[SH::ITL][SH::RL];
 #pragma unroll
   for (int s = 0; s < SH::ITL; ++s)
@@ -431,14 +489,15 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   float gcache[SH::ITL][SH::RL];
   int g_l = -1;
   auto gauss_fill = [&](int r) {
-    const float a = gld(J.row_a, 4u * (unsigned)r);
+    const float a = row_c(r).x;
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
       for (int q = 0; q < SH::RL; ++q) gcache[s][q] = __expf(-(a * xs[s][q] * xs[s][q]));
   };
   auto row_ft = [&](int r, float (&F)[SH::ITL][SH::RL]) {
-    const float bco = gld(J.row_b, 4u * (unsigned)r);
+    const float4 c = row_c(r);
+    const float bco = c.y;
     const float E = __expf(-bco * dlt);
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s)
@@ -452,9 +511,9 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
           H *= E;
         }
       }
-    const int nf = gld(J.nfold, 4u * (unsigned)r);
+    const int nf = __float_as_int(c.w);
     if (nf > 0) {
-      const float a = gld(J.row_a, 4u * (unsigned)r);
+      const float a = c.x;
 #pragma unroll
       for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
@@ -474,7 +533,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
           if (s == w_s) F[s][SH::RL - 1] += hv;
       }
     }
-    const float f0 = gld(J.row_inv_f0, 4u * (unsigned)r);
+    const float f0 = c.z;
 #pragma unroll
     for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
@@ -505,7 +564,8 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
     auto next = [&] {
       if (p + 1 < P) prefetch(p + 1);
     };
-    fc_fft_io<N2, B, T, true, true, true>(buf, s_tw, ld, st, next);
+    auto nop = [] {};
+    fc_fft_io<N2, B, T, true, true, true, false, 4, true>(buf, s_tw, ld, st, next, nop);
   }
   __syncthreads();
   // the accumulated spectrum into LDS, in the forward last stage's element order
@@ -517,7 +577,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       const int k = j % NSL;
       const int e0 = (((j - k) * SH::RL + k) << LB) + b;
 #pragma unroll
-      for (int q = 0; q < SH::RL; ++q) buf[pad_idx(e0 + q * NSL * B)] = acc[s][q];
+      for (int q = 0; q < SH::RL; ++q) buf[pad_e<4>(e0 + q * NSL * B)] = acc[s][q];
     }
   }
   __syncthreads();
@@ -531,7 +591,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   };
   FcNone none;
   auto nop = [] {};
-  fc_fft_io<N2, B, T, false, false, true>(buf, s_tw, none, st_out, nop);
+  fc_fft_io<N2, B, T, false, false, true, false, 4, true>(buf, s_tw, none, st_out, nop, nop);
 }
 
 // ---------------------------------------------------------------------------
