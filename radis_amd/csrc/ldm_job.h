@@ -159,7 +159,7 @@ struct LdmJob {
   float qi_f[LDM_MAX_ISO];       // qratio * i_scale in FP32 (used when i_fast)
   int i_fast;                    // 1: every qi_f is well inside FP32's range
   double dxG, dxL, fold_thresh;
-  double shift_margin;           // max|delta_air|*p_atm + 2*delta_w
+  double shift_margin;           // max|delta_air|*p_atm + 1e-3*delta_w
   float c_stim_T, c_stim_R;      // -hc_k/T, -hc_k/Tref (stimulated emission, FP32)
   float log_r_f;                 // log(Tref/T), FP32
   float pa_air_f, pa_self_f;     // p_atm*(1-x), p_atm*x, FP32

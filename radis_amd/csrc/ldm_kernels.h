@@ -524,7 +524,7 @@ __device__ __forceinline__ long long lower_bound_nu_wave(const double* __restric
 // k_tile_bounds: for every fill tile [k_lo, k_hi) the line range that can
 // put a corner inside it: k0 in [k_lo-1, k_hi-1].  A line's np.interp
 // position is monotone in its shifted centre, so a search on nu0 widened by
-// the largest pressure shift (plus two grid steps for rounding) is a
+// the largest pressure shift (plus 1e-3 of a grid step for rounding) is a
 // superset; k_fill filters each line by its exact k0.  Item = (job, tile).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, int4* __restrict__ items,

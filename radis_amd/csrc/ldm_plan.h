@@ -676,7 +676,10 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.dxG = r.dxG;
   J.dxL = r.dxL;
   J.fold_thresh = r.folding_thresh;
-  J.shift_margin = (db.has_delta ? db.max_abs_delta * J.p_atm : 0.0) + 2 * R.delta_w;
+  // a line's index can round up across a node only from within a few ulp
+  // below it: 1e-3 of a step covers that; each step of margin would add
+  // 1/tile_w of the tile's line reads
+  J.shift_margin = (db.has_delta ? db.max_abs_delta * J.p_atm : 0.0) + 1e-3 * R.delta_w;
   J.log_r_f = (float)J.log_r;
   J.pa_air_f = (float)(J.p_atm * J.x_air);
   J.pa_self_f = (float)(J.p_atm * J.x_self);
