@@ -435,7 +435,10 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
     if (X.tile_bounds)
       launch(b, timed, slot, 2, k_tile_bounds, dim3((X.n_fill + 3) / 4), dim3(256), 0, s, X.jobs,
              X.items + (items - X.items), X.n_fill, dbd);
-    launch(b, timed, slot, 3, k_fill, dim3(X.n_fill), dim3(LDM_FILL_T), X.fill_lds, s, cj, ca, items, dbd);
+    if (!dbd.iso && !dbd.delta && !dbd.n_self)
+      launch(b, timed, slot, 3, k_fill<true>, dim3(X.n_fill), dim3(LDM_FILL_T), X.fill_lds, s, cj, ca, items, dbd);
+    else
+      launch(b, timed, slot, 3, k_fill<false>, dim3(X.n_fill), dim3(LDM_FILL_T), X.fill_lds, s, cj, ca, items, dbd);
   }
   items += X.n_fill;
   for (int g = 0; g < X.nga; ++g) {

@@ -935,6 +935,10 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   }
 }
 
+// One kernel per line-list variant (PLAIN: no isotopologue, shift or
+// n_self column; the host picks), so each gets its own register allocation:
+// the plain kernel needs the fewest.
+template <bool PLAIN>
 __global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
                                                      const int4* __restrict__ items, LdmDb db) {
@@ -943,14 +947,8 @@ __global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict
   const LdmJob& J = jobs[it.x];
   const LdmAxes& A = axes[it.x];
   if (A.m == 0 || J.status) return;
-  const bool plain = !db.iso && !db.delta && !db.n_self;
-  if (J.noneq) {
-    if (plain) fill_tile<true, true>(J, A, it, db, acc);
-    else fill_tile<true, false>(J, A, it, db, acc);
-  } else {
-    if (plain) fill_tile<false, true>(J, A, it, db, acc);
-    else fill_tile<false, false>(J, A, it, db, acc);
-  }
+  if (J.noneq) fill_tile<true, PLAIN>(J, A, it, db, acc);
+  else fill_tile<false, PLAIN>(J, A, it, db, acc);
 }
 
 // four-step twiddle W_L^e (forward sign)
