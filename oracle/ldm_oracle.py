@@ -507,13 +507,6 @@ def splev_fitpack(t, c, k, x):
     return np.asarray(out).reshape(np.shape(x))
 
 
-def residual_L2(a, b):
-    """RADIS ``get_residual`` norm ``sqrt(sum(dI**2))/N``; spectrum/compare.py:418-419."""
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    return float(np.sqrt(np.sum((a - b) ** 2)) / a.size)
-
-
 def rel_peak_error(test, ref):
     """max |test - ref| / max |ref| — the parity metric used for every gate."""
     ref = np.asarray(ref, dtype=np.float64)

@@ -129,6 +129,21 @@ def test_folding_terms_brute_force():
     np.testing.assert_allclose(T, tot / tot[0], rtol=1e-12)
 
 
+def test_fold_count_closed_forms():
+    """Pure Lorentzian: V(n/(2 ws)) = exp(-pi n hL / ws) >= t for
+    n <= ws ln(1/t) / (pi hL); pure Gaussian: exp(-(pi n hG / ws)^2 / (4 ln 2))
+    >= t for n <= ws sqrt(4 ln2 ln(1/t)) / (pi hG).  An off-by-one in the
+    loop (counting the first failing alias) breaks both."""
+    t = o.FOLDING_THRESH_DEFAULT
+    for ws, h in ((0.05, 0.003), (0.01, 0.0007), (0.002, 0.0001), (0.1, 0.02)):
+        nL = int(np.floor(ws * np.log(1 / t) / (np.pi * h)))
+        nG = int(np.floor(ws * np.sqrt(4 * np.log(2) * np.log(1 / t)) / (np.pi * h)))
+        assert o.fold_count(ws, 0.0, h) == nL
+        assert o.fold_count(ws, h, 0.0) == nG
+    # no alias term when the FT has decayed by the first alias
+    assert o.fold_count(0.001, 0.01, 0.01) == 0
+
+
 # --- scatter and FFT apply: brute force on tiny inputs -------------------------------
 def test_ldm_scatter_brute_force():
     rng = np.random.default_rng(4)
