@@ -57,6 +57,17 @@ constexpr int pad_ce(int c) {
   return PD == 5 ? pad_c(c) : c + (c >> 4);
 }
 
+// Per-stage twiddle tables (fc_load_tw_stages): stage s >= 1 (radix R, NS)
+// holds W^(q k), q in [1, R), k < NS, as (q-1)*NS + k; a stage with
+// NS >= 64 holds only q = 1 (NS entries) and its butterflies form the
+// powers W^(q k) = W^(a k) W^(b k), a + b = q, a = q/2 (products at most
+// log2(R) deep).  fc_tw_off is a stage's offset in the concatenated table.
+constexpr bool fc_tw_compact(int n, int s) { return s > 0 && fc_ns(n, s) >= 64; }
+constexpr int fc_tw_size(int n, int s) {
+  return s == 0 ? 0 : (fc_tw_compact(n, s) ? fc_ns(n, s) : fc_ns(n, s) * (fc_radix(n, s) - 1));
+}
+constexpr int fc_tw_off(int n, int s) { return s <= 1 ? 0 : fc_tw_off(n, s - 1) + fc_tw_size(n, s - 1); }
+
 // One Stockham DIT stage (see ldm_fft.h for the recursion) with compile-time
 // N, radix R, Ns, B and T.  Element e = i*B + b is element i of column b.
 // IN_REG: the stage's inputs come from ld(s, q, e) instead of LDS (the first
@@ -75,12 +86,12 @@ __device__ __forceinline__ int fc_tid() {
 }
 
 // PD: the LDS padding (pad_e).  TWO >= 0: stage twiddles from a per-stage
-// table (fc_load_tw_stages) at tw[TWO + (q-1)*NS + k], so lanes with
-// consecutive k read consecutive slots; TWO < 0: from the W_N table at
-// tw[q*k*N/(NS*R)], whose power-of-two strides put a wave's lanes on a few
-// LDS banks when B is small.
-template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD, int TWO, class LD,
-          class ST>
+// table (fc_load_tw_stages) at tw[TWO + (q-1)*NS + k] (TWC: only q = 1,
+// powers formed here), so lanes with consecutive k read consecutive slots;
+// TWO < 0: from the W_N table at tw[q*k*N/(NS*R)], whose power-of-two
+// strides put a wave's lanes on a few LDS banks when B is small.
+template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD, int TWO, bool TWC,
+          class LD, class ST>
 __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st) {
   const int tid = OPQ ? fc_tid() : (int)threadIdx.x;
   constexpr int NR = N / R;
@@ -113,10 +124,19 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
         const int j = it >> LB;
         const int k = j % NS;
         const int e = k * (N / (NS * R));
+        if constexpr (TWO >= 0 && TWC) {
+          float2 w[R];
+          w[1] = tw[TWO + k];
 #pragma unroll
-        for (int q = 1; q < R; ++q) {
-          const float2 w = TWO >= 0 ? tw[TWO + (q - 1) * NS + k] : tw[q * e];
-          v[s][q] = FWD ? cmul(v[s][q], w) : cmulc(v[s][q], w);
+          for (int q = 2; q < R; ++q) w[q] = cmul(w[q / 2], w[q - q / 2]);
+#pragma unroll
+          for (int q = 1; q < R; ++q) v[s][q] = FWD ? cmul(v[s][q], w[q]) : cmulc(v[s][q], w[q]);
+        } else {
+#pragma unroll
+          for (int q = 1; q < R; ++q) {
+            const float2 w = TWO >= 0 ? tw[TWO + (q - 1) * NS + k] : tw[q * e];
+            v[s][q] = FWD ? cmul(v[s][q], w) : cmulc(v[s][q], w);
+          }
         }
       }
       dft<R, FWD>(v[s]);
@@ -167,9 +187,9 @@ __device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2
   constexpr int NST = fc_nstages(N);
   if constexpr (S < NST) {
     if constexpr (S == NST - 1) before_last();
-    constexpr int TWO = (TWS && S > 0) ? fc_ns(N, S) - fc_ns(N, 1) : -1;
-    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ, PD, TWO>(
-        buf, tw, ld, st);
+    constexpr int TWO = (TWS && S > 0) ? fc_tw_off(N, S) : -1;
+    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ, PD, TWO,
+                fc_tw_compact(N, S)>(buf, tw, ld, st);
     if constexpr (S == 0) after_first();
     fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, S + 1>(buf, tw, ld, st, after_first, before_last);
   }
@@ -205,16 +225,16 @@ __device__ __forceinline__ void fc_load_tw(float2* __restrict__ s_tw, const floa
 }
 
 // Per-stage twiddle tables from the W_N table tw (global): stage
-// S >= 1 (radix R, NS) holds W_N^(q k N/(NS R)), q in [1, R), k < NS, at
-// fc_ns(N, S) - fc_ns(N, 1) + (q-1)*NS + k.  The stages' tables add up to
-// N - fc_ns(N, 1) < N entries.
+// S >= 1 (radix R, NS) holds W_N^(q k N/(NS R)) for k < NS and q in
+// [1, R) (q = 1 only when fc_tw_compact) at fc_tw_off(N, S) + (q-1)*NS + k.
+// The tables add up to fewer than N entries (fc_tw_off(N, fc_nstages(N))).
 template <int N, int T, int S = 1>
 __device__ __forceinline__ void fc_load_tw_stages(float2* __restrict__ s_tw, const float2* __restrict__ tw) {
   if constexpr (S < fc_nstages(N)) {
     constexpr int NS = fc_ns(N, S), R = fc_radix(N, S), LNS = fc_log2(NS);
-    constexpr int O = NS - fc_ns(N, 1), STEP = N / (NS * R);
+    constexpr int O = fc_tw_off(N, S), STEP = N / (NS * R);
     static_assert(fc_pow2(NS), "stages after the first have power-of-two NS");
-    for (int i = threadIdx.x; i < NS * (R - 1); i += T) {
+    for (int i = threadIdx.x; i < fc_tw_size(N, S); i += T) {
       const int q = (i >> LNS) + 1, k = i & (NS - 1);
       s_tw[O + i] = gld(tw, 8u * (unsigned)(q * k * STEP));
     }
@@ -377,8 +397,13 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   using SH = FcShape<N2, B, T>;
   constexpr int NR0 = N2 / SH::R0;
   constexpr int NSL = fc_ns(N2, fc_nstages(N2) - 1);
-  __shared__ float2 buf[LdsSize<CAP>::value];
-  __shared__ float2 s_tw[N2];
+  // two transform buffers: pair p uses bufs[p & 1], so the next pair's
+  // first stage can write while slower waves still read the last stage of
+  // this pair (the other buffer's last readers passed the previous pair's
+  // stage barriers)
+  __shared__ float2 bufs[2][LdsSize<CAP>::value];
+  float2* const buf = bufs[0];
+  __shared__ float2 s_tw[fc_tw_off(N2, fc_nstages(N2))];  // per-stage twiddle tables
   const int4 it = items[blockIdx.x];
   const LdmJob& J = jobs[it.x];
   if (J.status) return;
@@ -541,8 +566,6 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   };
   if (P > 0) prefetch(0);
   for (int p = 0; p < P; ++p) {
-    // the previous pair's last stage has finished reading LDS
-    if (p > 0) __syncthreads();
     const int ra = 2 * p, rb = 2 * p + 1;
     float fa[SH::ITL][SH::RL], fb[SH::ITL][SH::RL];
     if (ra / NLr != g_l) { g_l = ra / NLr; gauss_fill(ra); }
@@ -565,7 +588,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       if (p + 1 < P) prefetch(p + 1);
     };
     auto nop = [] {};
-    fc_fft_io<N2, B, T, true, true, true, false, 4, true>(buf, s_tw, ld, st, next, nop);
+    fc_fft_io<N2, B, T, true, true, true, false, 4, true>(bufs[p & 1], s_tw, ld, st, next, nop);
   }
   __syncthreads();
   // the accumulated spectrum into LDS, in the forward last stage's element order
