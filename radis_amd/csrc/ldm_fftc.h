@@ -90,9 +90,12 @@ __device__ __forceinline__ int fc_tid() {
 // powers formed here), so lanes with consecutive k read consecutive slots;
 // TWO < 0: from the W_N table at tw[q*k*N/(NS*R)], whose power-of-two
 // strides put a wave's lanes on a few LDS banks when B is small.
+// OOP: the stage writes obuf, a buffer other than the one it reads (buf),
+// so no barrier is needed between its reads and its writes.
 template <int N, int B, int T, int R, int NS, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD, int TWO, bool TWC,
-          class LD, class ST>
-__device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st) {
+          bool OOP, class LD, class ST>
+__device__ __forceinline__ void fc_stage_io(const float2* buf, float2* obuf, const float2* __restrict__ tw, LD& ld,
+                                            ST& st) {
   const int tid = OPQ ? fc_tid() : (int)threadIdx.x;
   constexpr int NR = N / R;
   constexpr int ITEMS = NR * B;
@@ -143,7 +146,7 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
     }
   }
   // LDS inputs must all be read before any output overwrites them
-  if constexpr (!IN_REG && !OUT_REG) __syncthreads();
+  if constexpr (!IN_REG && !OUT_REG && !OOP) __syncthreads();
 #pragma unroll
   for (int s = 0; s < IT; ++s) {
     const int it = tid + s * T;
@@ -155,12 +158,12 @@ __device__ __forceinline__ void fc_stage_io(float2* __restrict__ buf, const floa
 #pragma unroll
         for (int q = 0; q < R; ++q) st(s, q, e0 + q * NS * B, v[s][q]);
       } else if constexpr (WR_LIN) {
-        float2* dst = buf + pad_e<PD>(e0);
+        float2* dst = obuf + pad_e<PD>(e0);
 #pragma unroll
         for (int q = 0; q < R; ++q) dst[pad_ce<PD>(q * NS * B)] = v[s][q];
       } else {
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[pad_e<PD>(e0 + q * NS * B)] = v[s][q];
+        for (int q = 0; q < R; ++q) obuf[pad_e<PD>(e0 + q * NS * B)] = v[s][q];
       }
     }
   }
@@ -179,20 +182,30 @@ struct FcNone {
 // OPQ: the stages take the thread index through fc_tid() (for callers that
 // run the transform inside a loop).
 // PD: the LDS padding (pad_e).  TWS: tw is the per-stage twiddle table
-// (fc_load_tw_stages) instead of W_N.
-template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD = 5, bool TWS = false, int S = 0,
-          class LD, class ST, class AF, class BL>
-__device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
+// (fc_load_tw_stages) instead of W_N.  Ping-pong (b1 != b0): stage S writes
+// b0 for even S and b1 for odd S and reads the other one, so no stage needs
+// a barrier between its reads and its writes; b1 == b0: in place.
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD = 5, bool TWS = false,
+          bool PP = false, int S = 0, class LD, class ST, class AF, class BL>
+__device__ __forceinline__ void fc_fft_pp(float2* b0, float2* b1, const float2* __restrict__ tw, LD& ld, ST& st,
                                           AF& after_first, BL& before_last) {
   constexpr int NST = fc_nstages(N);
   if constexpr (S < NST) {
     if constexpr (S == NST - 1) before_last();
     constexpr int TWO = (TWS && S > 0) ? fc_tw_off(N, S) : -1;
+    float2* ib = (PP && (S & 1) == 0) ? b1 : b0;  // stage S reads what stage S-1 wrote
+    float2* ob = (PP && (S & 1)) ? b1 : b0;
     fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ, PD, TWO,
-                fc_tw_compact(N, S)>(buf, tw, ld, st);
+                fc_tw_compact(N, S), PP>(ib, ob, tw, ld, st);
     if constexpr (S == 0) after_first();
-    fc_fft_io<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, S + 1>(buf, tw, ld, st, after_first, before_last);
+    fc_fft_pp<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, PP, S + 1>(b0, b1, tw, ld, st, after_first, before_last);
   }
+}
+template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD = 5, bool TWS = false, class LD,
+          class ST, class AF, class BL>
+__device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
+                                          AF& after_first, BL& before_last) {
+  fc_fft_pp<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, false>(buf, buf, tw, ld, st, after_first, before_last);
 }
 template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, class LD, class ST, class AF>
 __device__ __forceinline__ void fc_fft_io(float2* __restrict__ buf, const float2* __restrict__ tw, LD& ld, ST& st,
@@ -397,12 +410,14 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   using SH = FcShape<N2, B, T>;
   constexpr int NR0 = N2 / SH::R0;
   constexpr int NSL = fc_ns(N2, fc_nstages(N2) - 1);
-  // two transform buffers: pair p uses bufs[p & 1], so the next pair's
-  // first stage can write while slower waves still read the last stage of
-  // this pair (the other buffer's last readers passed the previous pair's
-  // stage barriers)
+  // Two transform buffers, used ping-pong (stage s writes one, stage s+1
+  // reads it and writes the other): no stage needs a barrier between its
+  // reads and its writes, and the next pair's first stage writes a buffer
+  // whose last readers have passed a stage barrier.  With an even number of
+  // stages the last stage reads the first stage's buffer, so the pairs
+  // alternate the two roles.
   __shared__ float2 bufs[2][LdsSize<CAP>::value];
-  float2* const buf = bufs[0];
+  constexpr bool SWAP = fc_nstages(N2) % 2 == 0;
   __shared__ float2 s_tw[fc_tw_off(N2, fc_nstages(N2))];  // per-stage twiddle tables
   const int4 it = items[blockIdx.x];
   const LdmJob& J = jobs[it.x];
@@ -588,10 +603,12 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       if (p + 1 < P) prefetch(p + 1);
     };
     auto nop = [] {};
-    fc_fft_io<N2, B, T, true, true, true, false, 4, true>(bufs[p & 1], s_tw, ld, st, next, nop);
+    float2* const x0 = bufs[SWAP ? (p & 1) : 0];
+    float2* const x1 = bufs[SWAP ? 1 - (p & 1) : 1];
+    fc_fft_pp<N2, B, T, true, true, true, false, 4, true, true>(x0, x1, s_tw, ld, st, next, nop);
   }
   __syncthreads();
-  // the accumulated spectrum into LDS, in the forward last stage's element order
+  // the accumulated spectrum into LDS (bufs[1]), in the forward last stage's element order
 #pragma unroll
   for (int s = 0; s < SH::ITL; ++s) {
     const int t = threadIdx.x + s * T;
@@ -600,7 +617,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       const int k = j % NSL;
       const int e0 = (((j - k) * SH::RL + k) << LB) + b;
 #pragma unroll
-      for (int q = 0; q < SH::RL; ++q) buf[pad_e<4>(e0 + q * NSL * B)] = acc[s][q];
+      for (int q = 0; q < SH::RL; ++q) bufs[1][pad_e<4>(e0 + q * NSL * B)] = acc[s][q];
     }
   }
   __syncthreads();
@@ -614,7 +631,8 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   };
   FcNone none;
   auto nop = [] {};
-  fc_fft_io<N2, B, T, false, false, true, false, 4, true>(buf, s_tw, none, st_out, nop, nop);
+  // first stage reads bufs[1] (ping-pong)
+  fc_fft_pp<N2, B, T, false, false, true, false, 4, true, true>(bufs[0], bufs[1], s_tw, none, st_out, nop, nop);
 }
 
 // ---------------------------------------------------------------------------
