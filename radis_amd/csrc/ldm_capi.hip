@@ -1062,9 +1062,11 @@ int radis_ldm_batch_copy_ldm(radis_ldm_batch* b, int i, float* dst, void* stream
   if (rc) return rc;
   const int rows = b->res_h[b->last_slot][i].ng * b->res_h[b->last_slot][i].nl;
   if (rows == 0) return 0;
-  const size_t cols = (size_t)(R.nv + 1);
-  HIPCHK(hipMemcpy2DAsync(dst, cols * 4, (char*)b->work + R.off_ldm, (size_t)R.stride * 4, cols * 4, rows,
-                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  const long long cols = R.nv + 1;
+  const long long n = (long long)rows * cols;
+  hipLaunchKernelGGL(k_copy_ldm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const float*)((char*)b->work + R.off_ldm), (long long)R.stride, rows, cols, dst);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -318,9 +318,9 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
     const int p = it.y;
     if (J.status || p >= axes[it.x].p) return false;
     const int N2 = J.n2, n2_0 = it.z * B, V = (int)J.v;
-    const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
+    // pair p's rows, interleaved: z = (x_2p, x_2p+1) is one float2
+    const float2* rz = reinterpret_cast<const float2*>(J.ldm) + (long long)p * J.ldm_stride;
     const bool has_b = (2 * p + 1) < axes[it.x].m;
-    const float* rb = has_b ? ra + J.ldm_stride : ra;
     // L >= 2 N_v, so V = N_v + 1 <= L/2 unless L = 2 N_v exactly: then the
     // upper half of every column (n1 >= N1/2, n >= L/2) is zero and its
     // loads are skipped
@@ -339,9 +339,9 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
         const int e = it0 + q * NR0 * B;
         const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);
         const bool in = (SH::ITEMS0 % T == 0 || it0 < SH::ITEMS0) && n < V;
-        const unsigned o = in ? 4u * (unsigned)n : 0u;
-        const float xa = gld(ra, o), xb = gld(rb, o);
-        v[s][q] = make_float2(in ? xa : 0.0f, (in && has_b) ? xb : 0.0f);
+        const unsigned o = in ? 8u * (unsigned)n : 0u;  // n < V < 2^28
+        const float2 z = gld(rz, o);
+        v[s][q] = make_float2(in ? z.x : 0.0f, (in && has_b) ? z.y : 0.0f);
       }
     }
     return true;

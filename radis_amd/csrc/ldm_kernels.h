@@ -912,26 +912,24 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     }
   }
   __syncthreads();
-  // flush: W is a power of two and LDS rows are 16-byte aligned; a lane owns
-  // 4 columns and walks the rows (W/4 >= T: one row at a time over several
-  // column groups; W/4 < T: T/(W/4) rows at once)
-  const int W4 = W >> 2, sh = __builtin_ctz(W4);
-  const bool wide = W4 >= LDM_FILL_T;
-  const int r0 = wide ? 0 : (int)(threadIdx.x >> sh);
-  const int rstep = wide ? 1 : (LDM_FILL_T >> sh);
-  const long long dstep = (long long)rstep * J.ldm_stride;
-  for (int c = 4 * (int)(threadIdx.x & (W4 - 1)); c < wlen; c += 4 * LDM_FILL_T) {
-    float* dst = J.ldm + (k_lo + c) + (long long)r0 * J.ldm_stride;
-    const double* src = acc + r0 * Wp + c;
-    if (c + 4 <= wlen) {
-      for (int r = r0; r < M; r += rstep, dst += dstep, src += rstep * Wp) {
-        const double2 a = reinterpret_cast<const double2*>(src)[0], b = reinterpret_cast<const double2*>(src)[1];
-        *reinterpret_cast<float4*>(dst) = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
-      }
-    } else {
-      for (int r = r0; r < M; r += rstep, dst += dstep, src += rstep * Wp)
-        for (int q = 0; q < wlen - c; ++q) dst[q] = (float)src[q];
-    }
+  // flush: the LDM interleaves rows 2p and 2p+1 (element n of row 2p+h at
+  // float 2n+h of pair p).  A lane writes 2 columns of one pair (16 bytes)
+  // per step: (2p, c), (2p+1, c), (2p, c+1), (2p+1, c+1); the second row of
+  // an odd last pair is written as zeros.  W is a power of two and LDS rows
+  // are 16-byte aligned.
+  const int P2 = (M + 1) >> 1;
+  const int W2 = W >> 1, sh2 = __builtin_ctz(W2);
+  const long long pstride = 2 * J.ldm_stride;  // floats per pair
+  float* const out = J.ldm + 2 * k_lo;          // column k_lo of pair 0
+  for (int w = threadIdx.x; w < (P2 << sh2); w += LDM_FILL_T) {
+    const int p = w >> sh2, c = 2 * (w & (W2 - 1));
+    if (c >= wlen) continue;
+    const double* src = acc + (2 * p) * Wp + c;
+    const double2 a = *reinterpret_cast<const double2*>(src);
+    const double2 b = (2 * p + 1 < M) ? *reinterpret_cast<const double2*>(src + Wp) : make_double2(0.0, 0.0);
+    float* dst = out + p * pstride + 2 * c;
+    if (c + 2 <= wlen) *reinterpret_cast<float4*>(dst) = make_float4((float)a.x, (float)b.x, (float)a.y, (float)b.y);
+    else *reinterpret_cast<float2*>(dst) = make_float2((float)a.x, (float)b.x);
   }
 }
 
@@ -984,17 +982,17 @@ __global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_a(const LdmJob* __restrict
   if (J.status || p >= axes[it.x].p) return;
   const int N1 = J.n1, N2 = J.n2, bs = J.bA, B = 1 << bs;
   const long long n2_0 = (long long)it.z * B;
-  const float* ra = J.ldm + (long long)(2 * p) * J.ldm_stride;
+  const float2* rz = reinterpret_cast<const float2*>(J.ldm) + (long long)p * J.ldm_stride;  // rows 2p, 2p+1
   const bool has_b = (2 * p + 1) < axes[it.x].m;
-  const float* rb = ra + J.ldm_stride;
   const long long V = J.v;
   for (int e = threadIdx.x; e < (N1 << bs); e += LDM_FFT_T) {
     const int n1 = e >> bs, b = e & (B - 1);
     const long long n = n2_0 + b + (long long)N2 * n1;
     float xa = 0.0f, xb = 0.0f;
     if (n < V) {
-      xa = ra[n];
-      if (has_b) xb = rb[n];
+      const float2 z = rz[n];
+      xa = z.x;
+      if (has_b) xb = z.y;
     }
     buf[pad_idx(e)] = make_float2(xa, xb);
   }

@@ -779,7 +779,9 @@ int layout_request(Req& R, size_t& total) {
     R.bC = R.bA;
   }
   const int pcap = (R.m_cap + 1) / 2;
-  R.off_ldm = total; total = align_up(total + (size_t)R.m_cap * R.stride * 4, 256);
+  // LDM: pcap pairs of rows, rows 2p and 2p+1 interleaved (element n of row
+  // 2p+h at float 2n+h of pair p): pass A reads each element pair as one float2
+  R.off_ldm = total; total = align_up(total + (size_t)pcap * 2 * R.stride * 4, 256);
   R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
   R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
   R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
@@ -797,7 +799,7 @@ int layout_request(Req& R, size_t& total) {
 int check_layout(const Req& R, size_t total) {
   const size_t V = (size_t)R.nv + 1;
   const size_t parts = (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2);
-  const bool ok = (size_t)R.stride >= V && R.off_ldm + (size_t)R.m_cap * R.stride * 4 <= total &&
+  const bool ok = (size_t)R.stride >= V && R.off_ldm + (size_t)((R.m_cap + 1) / 2) * 2 * R.stride * 4 <= total &&
                   R.off_y + (size_t)((R.m_cap + 1) / 2) * R.plan.L * 8 <= total &&
                   R.off_b + (size_t)R.plan.L * 8 <= total && R.off_rows + (size_t)R.m_cap * 16 <= total &&
                   R.off_axp + parts * 32 <= total && (size_t)R.n_axp <= parts &&

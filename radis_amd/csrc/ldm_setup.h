@@ -50,3 +50,14 @@ __global__ __launch_bounds__(64) void k_put_job(LdmJob job, LdmJob* __restrict__
   unsigned* d = (unsigned*)dst;
   for (int w = threadIdx.x; w < (int)(sizeof(LdmJob) / 4); w += blockDim.x) d[w] = s[w];
 }
+
+// The first `rows` LDM rows of a request de-interleaved into dst[rows][cols]
+// (the LDM stores rows 2p and 2p+1 interleaved; a diagnostic copy).
+__global__ __launch_bounds__(256) void k_copy_ldm(const float* __restrict__ ldm, long long stride, int rows,
+                                                  long long cols, float* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)rows * cols) return;
+  const int r = (int)(i / cols);
+  const long long n = i - (long long)r * cols;
+  dst[i] = ldm[(long long)(r >> 1) * 2 * stride + 2 * n + (r & 1)];
+}
