@@ -281,11 +281,12 @@ struct TwProg {
 // ---------------------------------------------------------------------------
 // Pass A with N1 fixed (see k_fft_a): B = fc_cols_a(N1) columns per workgroup.
 // ---------------------------------------------------------------------------
-// Pass A reads B consecutive floats of each of a column block's N1 rows
-// (stride N2): row segments of 128 B or more (B >= 32) stream at close to
-// the full HBM rate, 64 B ones at about 0.6 of it and 32 B ones at about
-// 0.35, so pass A takes the widest tile that fits LDS (one 1024-thread
-// workgroup per CU).
+// Pass A reads B consecutive elements of each of a column block's N1 rows
+// (stride N2); an element is the float2 of the pair's two interleaved LDM
+// rows, so a row segment is 8 B bytes.  Segments of 128 B or more stream at
+// close to the full HBM rate, 64 B ones at about 0.6 of it and 32 B ones at
+// about 0.35, so pass A takes the widest tile that fits LDS (one
+// 1024-thread workgroup per CU).
 // (tile sizes FC_CAP_A, FC_CAP, FC_CAP_B and fc_cols_a / fc_cols_c /
 // fc_rows_b are in ldm_job.h: the host planner sizes the work items with them)
 
@@ -300,11 +301,12 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
   __shared__ float2 buf[LdsSize<CAPE>::value];
   __shared__ float2 s_tw[N1];
   // Persistent: workgroup g takes the column blocks g', g' + G, g' + 2G, ...
-  // (G = gridDim.x <= n_items).  Blocks narrower than a 128-B line take
+  // (G = gridDim.x <= n_items).  Blocks of fewer than 32 columns take
   // g' = xcd_block(g, G): the workgroups of one XCD then take consecutive
-  // blocks, so the two halves of a line meet in one L2 (N1 = 640, 100 M
-  // lines: 339 -> 330 us); full-line blocks keep g' = g (the remap cost
-  // them 2.6 us at N1 = 512).  A block's first-stage inputs
+  // blocks.  That was worth 339 -> 330 us at N1 = 640 when a 16-column
+  // block read 64-B row segments; with the LDM's interleaved row pairs it
+  // reads 128 B per row and the remap measures neutral.  32-column blocks
+  // keep g' = g (the remap cost them 2.6 us at N1 = 512).  A block's first-stage inputs
   // sit in v; the next block's loads go into v right after the first stage
   // has consumed them, so they are in flight during the LDS stages and the
   // stores.
