@@ -427,18 +427,22 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
   if (X.n_axis) launch(b, timed, slot, 0, k_axis, dim3(X.n_axis), dim3(256), 0, s, X.jobs, items, dbd);
   launch(b, timed, slot, 1, k_finalize, dim3(X.n_req), dim3(256), 0, s, X.jobs, X.axes, X.res, dbd);
   items += X.n_axis;
-  // (a fork of k_wrap onto a stream of its own, joined before pass B, was
-  // measured: the cross-stream events cost more than the kernel)
-  if (X.n_wrap) launch(b, timed, slot, 7, k_wrap, dim3(X.n_wrap), dim3(LDM_WRAP_T), 0, s, cj, ca, items);
+  // The image-correction items run as the first workgroups of the fill
+  // launch, concurrently with the tiles (as a launch of their own they held
+  // the device for ~11 us; a fork onto a stream of its own, joined before
+  // pass B, cost more in cross-stream events than the kernel).
+  const int4* wrap_fill = items;  // wrap items, then fill items
   items += X.n_wrap;
-  if (X.n_fill) {
-    if (X.tile_bounds)
-      launch(b, timed, slot, 2, k_tile_bounds, dim3((X.n_fill + 3) / 4), dim3(256), 0, s, X.jobs,
-             X.items + (items - X.items), X.n_fill, dbd);
+  if (X.n_fill && X.tile_bounds)
+    launch(b, timed, slot, 2, k_tile_bounds, dim3((X.n_fill + 3) / 4), dim3(256), 0, s, X.jobs,
+           X.items + (items - X.items), X.n_fill, dbd);
+  if (X.n_wrap + X.n_fill) {
+    const size_t lds = X.n_wrap ? std::max(X.fill_lds, (size_t)LDM_WRAP_LDS * sizeof(float2)) : X.fill_lds;
+    const dim3 grid(X.n_wrap + X.n_fill);
     if (!dbd.iso && !dbd.delta && !dbd.n_self)
-      launch(b, timed, slot, 3, k_fill<true>, dim3(X.n_fill), dim3(LDM_FILL_T), X.fill_lds, s, cj, ca, items, dbd);
+      launch(b, timed, slot, 3, k_fill<true>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
     else
-      launch(b, timed, slot, 3, k_fill<false>, dim3(X.n_fill), dim3(LDM_FILL_T), X.fill_lds, s, cj, ca, items, dbd);
+      launch(b, timed, slot, 3, k_fill<false>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
   }
   items += X.n_fill;
   for (int g = 0; g < X.nga; ++g) {

@@ -933,21 +933,8 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   }
 }
 
-// One kernel per line-list variant (PLAIN: no isotopologue, shift or
-// n_self column; the host picks), so each gets its own register allocation:
-// the plain kernel needs the fewest.
-template <bool PLAIN>
-__global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict__ jobs,
-                                                     const LdmAxes* __restrict__ axes,
-                                                     const int4* __restrict__ items, LdmDb db) {
-  extern __shared__ __align__(16) double acc[];
-  const int4 it = items[blockIdx.x];
-  const LdmJob& J = jobs[it.x];
-  const LdmAxes& A = axes[it.x];
-  if (A.m == 0 || J.status) return;
-  if (J.noneq) fill_tile<true, PLAIN>(J, A, it, db, acc);
-  else fill_tile<false, PLAIN>(J, A, it, db, acc);
-}
+// (the fill kernel, k_fill, is in ldm_wrap.h: its launch also runs the
+// image-correction items)
 
 // four-step twiddle W_L^e (forward sign)
 __device__ __forceinline__ float2 twiddle_L(const LdmJob& J, long long e) {

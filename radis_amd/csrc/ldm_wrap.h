@@ -23,12 +23,15 @@
 // equals the full one up to aliases of H_m beyond LC - K, far below 1e-9 of
 // it).
 //
-// One 1024-thread workgroup per (request, pair of L-axis nodes): the two
-// real sequences are the real and imaginary parts of one complex transform
-// (one or two samples per thread; the stage twiddles come from a table
-// computed with the batch's other twiddles).
+// One workgroup per (request, pair of L-axis nodes): the two real
+// sequences are the real and imaginary parts of one complex transform (the
+// stage twiddles come from a table computed with the batch's other
+// twiddles).  The items run as the first workgroups of the fill launch
+// (k_fill below), 256 threads each, concurrently with the fill tiles: pass
+// B, two launches later, is the first reader of the table.
 #pragma once
 #include "ldm_fftc.h"
+#include "ldm_kernels.h"
 
 struct WrapNode {
   double omE, omE2, E;  // 1 - e^-a, 1 - e^-2a, e^-a for the period's a = 2 pi gamma / period
@@ -49,13 +52,15 @@ __device__ __forceinline__ double lor_periodic(const WrapNode& w, double s2, dou
   return inv_period * w.omE2 / (w.omE * w.omE + 4.0 * w.E * s2);
 }
 
-#define LDM_WRAP_T 1024
+// LDS of one item: the transform buffer and its twiddle table (float2s)
+constexpr int LDM_WRAP_LDS = LdsSize<LDM_WRAP_LC>::value + LDM_WRAP_LC;
 
-// The table of one node pair from LC samples (LC >= 8 K: 512, 1024 or 2048).
-template <int LC>
+// The table of one node pair from LC samples (LC >= 8 K: 512, 1024 or 2048)
+// by T threads.
+template <int LC, int T>
 __device__ __forceinline__ void wrap_pair(const LdmJob& J, const LdmAxes& A, int ma, float2* __restrict__ buf,
                                           float2* __restrict__ s_tw) {
-  constexpr int T = LDM_WRAP_T, STRIDE = LDM_WRAP_LC / LC;
+  constexpr int STRIDE = LDM_WRAP_LC / LC;
   const int K = J.wrap_k;
   const int mb = ma + 1;
   const bool has_b = mb < A.nl;
@@ -104,19 +109,43 @@ __device__ __forceinline__ void wrap_pair(const LdmJob& J, const LdmAxes& A, int
   }
 }
 
-__global__ __launch_bounds__(LDM_WRAP_T) void k_wrap(const LdmJob* __restrict__ jobs,
-                                                     const LdmAxes* __restrict__ axes,
-                                                     const int4* __restrict__ items) {
-  __shared__ float2 buf[LdsSize<LDM_WRAP_LC>::value];
-  __shared__ float2 s_tw[LDM_WRAP_LC];
-  const int4 it = items[blockIdx.x];
+// One image-correction item (request it.x, nodes 2 it.y and 2 it.y + 1)
+// with T threads; lds holds LDM_WRAP_LDS float2s.
+template <int T>
+__device__ __forceinline__ void wrap_item(const LdmJob* __restrict__ jobs, const LdmAxes* __restrict__ axes,
+                                          const int4 it, float2* lds) {
   const LdmJob& J = jobs[it.x];
   const LdmAxes& A = axes[it.x];
   const int K = J.wrap_k;
   const int ma = 2 * it.y;
   if (J.status || K == 0 || ma >= A.nl) return;
+  float2* buf = lds;
+  float2* s_tw = lds + LdsSize<LDM_WRAP_LC>::value;
   // fewest samples with LC >= 8 K (uniform per workgroup)
-  if (8 * K <= 512) wrap_pair<512>(J, A, ma, buf, s_tw);
-  else if (8 * K <= 1024) wrap_pair<1024>(J, A, ma, buf, s_tw);
-  else wrap_pair<LDM_WRAP_LC>(J, A, ma, buf, s_tw);
+  if (8 * K <= 512) wrap_pair<512, T>(J, A, ma, buf, s_tw);
+  else if (8 * K <= 1024) wrap_pair<1024, T>(J, A, ma, buf, s_tw);
+  else wrap_pair<LDM_WRAP_LC, T>(J, A, ma, buf, s_tw);
+}
+
+// The fill launch: workgroups below n_wrap take the image-correction items
+// (items[0, n_wrap)), the others the fill tiles that follow them in the
+// item list.  One kernel per line-list variant (PLAIN: no isotopologue,
+// shift or n_self column; the host picks), so each gets its own register
+// allocation: the plain kernel needs the fewest.  The dynamic LDS holds the
+// larger of a fill tile and an image-correction item.
+template <bool PLAIN>
+__global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict__ jobs,
+                                                     const LdmAxes* __restrict__ axes,
+                                                     const int4* __restrict__ items, int n_wrap, LdmDb db) {
+  extern __shared__ __align__(16) double acc[];
+  const int4 it = items[blockIdx.x];
+  if ((int)blockIdx.x < n_wrap) {
+    wrap_item<LDM_FILL_T>(jobs, axes, it, reinterpret_cast<float2*>(acc));
+    return;
+  }
+  const LdmJob& J = jobs[it.x];
+  const LdmAxes& A = axes[it.x];
+  if (A.m == 0 || J.status) return;
+  if (J.noneq) fill_tile<true, PLAIN>(J, A, it, db, acc);
+  else fill_tile<false, PLAIN>(J, A, it, db, acc);
 }
