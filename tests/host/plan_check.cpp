@@ -139,7 +139,8 @@ void layout(const radis_ldm_db& db, HostBatch& B) {
     // own check, then each buffer here)
     CHECK(check_layout(R, B.total) == 0, "check_layout: %s", g_err);
     const size_t V = (size_t)R.nv + 1;
-    CHECK(R.off_ldm + (size_t)R.m_cap * R.stride * 4 <= B.total, "LDM");
+    // LDM: interleaved row pairs, ceil(m_cap / 2) pairs of 2 * stride floats
+    CHECK(R.off_ldm + (size_t)((R.m_cap + 1) / 2) * 2 * R.stride * 4 <= B.total, "LDM");
     CHECK((size_t)R.stride >= V, "LDM stride below N_v + 1");
     CHECK(R.off_y + (size_t)((R.m_cap + 1) / 2) * R.plan.L * 8 <= B.total, "Y");
     CHECK(R.off_b + (size_t)R.plan.L * 8 <= B.total, "B");
