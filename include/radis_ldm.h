@@ -223,7 +223,8 @@ void radis_ldm_batch_destroy(radis_ldm_batch *b);
  * fft_c, wrap) in milliseconds, averaged over the runs made with timing enabled
  * since the previous call; each stage is bracketed by start/stop events
  * (hipExtLaunchKernelGGL) on the launch stream.  A stage whose kernel did not
- * run (tile bounds reused) reports 0.
+ * run (tile bounds reused) reports 0; the image-correction items (wrap) run
+ * inside the fill launch, so wrap reports 0 and fill includes them.
  * Replaces GPUApplication.get_timestamps (vulkan_compute_lib.py:973-1024). */
 #define RADIS_LDM_N_STAGES 8
 int radis_ldm_batch_enable_timing(radis_ldm_batch *b, int enable);
