@@ -471,7 +471,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   };
   // frequencies of this thread's bins (forward last-stage element (s, q):
   // k2 = j + q*NS, j = (tid + s*T) >> LB), folded to [0, L/2]
-  float xs[SH::ITL][SH::RL], xrs[SH::ITL][SH::RL];
+  float xs[SH::ITL][SH::RL];
 #pragma unroll
   for (int s = 0; s < SH::ITL; ++s) {
     const int t = threadIdx.x + s * T;
@@ -480,7 +480,6 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       const long long k = (long long)(k1_0 + (t & (B - 1))) + (long long)N1 * ((t >> LB) + q * NSL);
       const long long kk = (k <= L - k) ? k : (L - k);
       xs[s][q] = (float)kk * J.inv_Lws;
-      xrs[s][q] = (float)(L / 2 - kk) * J.inv_Lws;
     }
   }
   // Lineshape FTs of this thread's bins.  F_r(x) = exp(-a_l x^2) exp(-b_m x)
@@ -561,7 +560,9 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
 #pragma unroll
         for (int q = 0; q < SH::RL; ++q)
           for (int n = 1; n <= nf; ++n) {
-            const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xrs[s][q]) : ((float)n * J.half_inv_ws + xs[s][q]);
+            // the folded frequency's mirror L/2 - kk (alias terms are rare: recomputed)
+            const float xr = (float)(L / 2) * J.inv_Lws - xs[s][q];
+            const float xa = (n & 1) ? ((float)n * J.half_inv_ws + xr) : ((float)n * J.half_inv_ws + xs[s][q]);
             F[s][q] += __expf(-(a * xa * xa) - bco * xa);
           }
     }
@@ -584,18 +585,22 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   if (P > 0) prefetch(0);
   for (int p = 0; p < P; ++p) {
     const int ra = 2 * p, rb = 2 * p + 1;
+    // the pair's lineshape FTs, evaluated just before the last stage that
+    // uses them (they are then not live across the earlier stages)
     float fa[SH::ITL][SH::RL], fb[SH::ITL][SH::RL];
-    if (ra / NLr != g_l) { g_l = ra / NLr; gauss_fill(ra); }
-    row_ft(ra, fa);
-    if (rb < M) {
-      if (rb / NLr != g_l) { g_l = rb / NLr; gauss_fill(rb); }
-      row_ft(rb, fb);
-    } else {
+    auto lineshapes = [&] {
+      if (ra / NLr != g_l) { g_l = ra / NLr; gauss_fill(ra); }
+      row_ft(ra, fa);
+      if (rb < M) {
+        if (rb / NLr != g_l) { g_l = rb / NLr; gauss_fill(rb); }
+        row_ft(rb, fb);
+      } else {
 #pragma unroll
-      for (int s = 0; s < SH::ITL; ++s)
+        for (int s = 0; s < SH::ITL; ++s)
 #pragma unroll
-        for (int q = 0; q < SH::RL; ++q) fb[s][q] = 0.0f;
-    }
+          for (int q = 0; q < SH::RL; ++q) fb[s][q] = 0.0f;
+      }
+    };
     auto ld = [&](int s, int q, int) -> float2 { return pre[s][q]; };
     auto st = [&](int s, int q, int, float2 z) {
       acc[s][q].x += z.x * fa[s][q] + z.y * fb[s][q];
@@ -604,10 +609,9 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
     auto next = [&] {
       if (p + 1 < P) prefetch(p + 1);
     };
-    auto nop = [] {};
     float2* const x0 = bufs[SWAP ? (p & 1) : 0];
     float2* const x1 = bufs[SWAP ? 1 - (p & 1) : 1];
-    fc_fft_pp<N2, B, T, true, true, true, false, 4, true, true>(x0, x1, s_tw, ld, st, next, nop);
+    fc_fft_pp<N2, B, T, true, true, true, false, 4, true, true>(x0, x1, s_tw, ld, st, next, lineshapes);
   }
   __syncthreads();
   // the accumulated spectrum into LDS (bufs[1]), in the forward last stage's element order
