@@ -166,7 +166,10 @@ void db_release(radis_ldm_db* db) {
 // One launch of an FFT pass: items of one kernel variant are contiguous;
 // cls = the compiled size (n1 for A and C, n2 for B), 0 = generic (pass B:
 // 0 = generic with small tiles, -1 = generic with large tiles).
-struct LaunchGroup { int cls, first, count; };
+struct LaunchGroup {
+  int cls, first, count;
+  int rows;  // pass B: the rows' coefficients staged in LDS (the largest m_cap of the group, at most FC_B_ROWS_LDS)
+};
 
 struct radis_ldm_batch {
   radis_ldm_ctx* ctx = nullptr;
@@ -234,6 +237,7 @@ int build_layout(radis_ldm_batch* b) {
   // --- workspace
   size_t total = 0;
   size_t fill_lds = 0;
+  choose_split_b(b->reqs.data(), (int)b->reqs.size(), b->ctx->n_cu);
   for (Req& R : b->reqs) {
     int rc = layout_request(R, total);
     if (rc) return rc;
@@ -289,6 +293,7 @@ int build_layout(radis_ldm_batch* b) {
   // the order plan_item gives them
   std::vector<int4> ax, fi, wr;
   std::map<int, std::vector<int4>> ma, mb, mc;
+  std::map<int, int> mb_rows;  // pass B class -> largest m_cap
   for (int j = 0; j < b->n; ++j) {
     Req& R = b->reqs[j];
     const LdmItemPlan P = item_plan(*b->db, R);
@@ -308,19 +313,21 @@ int build_layout(radis_ldm_batch* b) {
     put(fi, bf, ba);
     put(ma[ca_], ba, bb);
     put(mb[cb_], bb, bc);
-    put(mc[ca_], bc, be);
+    mb_rows[cb_] = std::max(mb_rows[cb_], R.m_cap);
+    put(mc[ca_ | (R.split_b > 1 ? FC_C_SUM : 0)], bc, be);  // split pass B: pass C adds two shares
   }
   std::vector<int4> fa, fb, fc;
   auto flatten = [](std::map<int, std::vector<int4>>& m, std::vector<int4>& v,
                     std::vector<radis_ldm_batch::Group>& g) {
     g.clear();
     for (auto& kv : m) {
-      g.push_back({kv.first, (int)v.size(), (int)kv.second.size()});
+      g.push_back({kv.first, (int)v.size(), (int)kv.second.size(), 0});
       v.insert(v.end(), kv.second.begin(), kv.second.end());
     }
   };
   flatten(ma, fa, b->grp_a);
   flatten(mb, fb, b->grp_b);
+  for (auto& G : b->grp_b) G.rows = std::min(mb_rows[G.cls], FC_B_ROWS_LDS);
   flatten(mc, fc, b->grp_c);
   b->n_axis = (int)ax.size();
   b->n_wrap = (int)wr.size();
@@ -467,7 +474,10 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
     const bool f = g == 0, l = g == X.ngb - 1;
     const dim3 grid(G.count);
 #define FC_B(N) \
-  case N: launch_part(b, timed, slot, 5, f, l, k_fft_b_c<N>, grid, dim3(FC_T_B), 0, s, cj, ca, it); break;
+  case N:                                                                                                    \
+    launch_part(b, timed, slot, 5, f, l, k_fft_b_c<N>, grid, dim3(FC_T_B), (size_t)G.rows * sizeof(float4), s, cj, \
+                ca, it, G.rows);                                                                               \
+    break;
     switch (G.cls) {
       FC_B(256) FC_B(512) FC_B(1024) FC_B(2048)
       case 0:
@@ -485,8 +495,11 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
     const int4* it = items + G.first;
     const bool f = g == 0, l = g == X.ngc - 1;
     const dim3 grid(G.count);
-#define FC_C(N) \
-  case N: launch_part(b, timed, slot, 6, f, l, k_fft_c_c<N>, grid, dim3(FC_T), 0, s, cj, it, G.count); break;
+#define FC_C(N)                                                                                                \
+  case N: launch_part(b, timed, slot, 6, f, l, k_fft_c_c<N, false>, grid, dim3(FC_T), 0, s, cj, it, G.count); break; \
+  case N | FC_C_SUM:                                                                                                  \
+    launch_part(b, timed, slot, 6, f, l, k_fft_c_c<N, true>, grid, dim3(FC_T), 0, s, cj, it, G.count);               \
+    break;
     switch (G.cls) {
       FC_C(256) FC_C(320) FC_C(384) FC_C(512) FC_C(640) FC_C(768) FC_C(1024)
       default: launch_part(b, timed, slot, 6, f, l, k_fft_c, grid, dim3(LDM_FFT_T), 0, s, cj, it, G.count);
@@ -788,6 +801,7 @@ int radis_ldm_request_bytes(const radis_ldm_db* db, const radis_ldm_request* req
   int rc = plan_one(*db, *req, R);
   if (rc) return rc;
   R.m_cap = R.m_need;
+  choose_split_b(&R, 1, db->ctx->n_cu);
   size_t total = 0;
   rc = layout_request(R, total);
   if (rc) return rc;
@@ -1088,6 +1102,7 @@ int radis_ldm_run_noalloc(radis_ldm_ctx* ctx, const radis_ldm_db* db, const radi
   int rc = plan_one(*db, *req, R);
   if (rc) return rc;
   R.m_cap = R.m_need;
+  choose_split_b(&R, 1, ctx->n_cu);
   size_t work = 0;
   rc = layout_request(R, work);
   if (rc) return rc;
@@ -1134,7 +1149,8 @@ int radis_ldm_run_noalloc(radis_ldm_ctx* ctx, const radis_ldm_db* db, const radi
   HIPCHK(hipGetLastError());
   const int cls_ac = R.plan.fc ? R.plan.n1 : 0;
   const int cls_b = R.plan.fc ? R.plan.n2 : (R.plan.n2 <= LDM_CAP_B ? 0 : -1);
-  const LaunchGroup ga{cls_ac, 0, P.pcap * P.n_colA}, gb{cls_b, 0, P.n_rowB}, gc{cls_ac, 0, P.n_colC};
+  const LaunchGroup ga{cls_ac, 0, P.pcap * P.n_colA, 0}, gb{cls_b, 0, P.n_rowB, std::min(R.m_cap, FC_B_ROWS_LDS)},
+      gc{cls_ac | (R.split_b > 1 ? FC_C_SUM : 0), 0, P.n_colC, 0};
   LaunchSet X{};
   X.jobs = (LdmJob*)(base + o_job);
   X.axes = (LdmAxes*)(base + o_axes);

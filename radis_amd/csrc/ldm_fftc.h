@@ -400,12 +400,15 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
 // registers (fetched during the previous pair) and its last stage multiplies
 // by the lineshape FTs and accumulates in registers; only the accumulated
 // spectrum goes through LDS into the inverse transform, whose last stage
-// writes B with the W_L twiddle.
+// writes B with the W_L twiddle.  Item (job, row block, share): with
+// J.bbuf2 set, share 0 takes the first ceil(P/2) pairs into B and share 1
+// the rest into bbuf2 (pass C adds the two).  The rows' lineshape
+// coefficients go to the dynamic LDS (n_rows_lds float4, launch-sized).
 // ---------------------------------------------------------------------------
 template <int N2>
 __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
                                                       const LdmAxes* __restrict__ axes,
-                                                      const int4* __restrict__ items) {
+                                                      const int4* __restrict__ items, int n_rows_lds) {
   constexpr int CAP = FC_CAP_B, T = FC_T_B;
   constexpr int B = fc_rows_b(N2), LB = fc_log2(B);
   static_assert(fc_pow2(N2), "pass B rows are powers of two");
@@ -428,13 +431,14 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
   const int k1_0 = it.y * B;
   const long long L = J.L;
   const int P = axes[it.x].p, M = axes[it.x].m;
+  const int p_mid = J.bbuf2 ? (P + 1) / 2 : P;
+  const int p0 = it.z ? p_mid : 0, p1 = it.z ? P : p_mid;  // this workgroup's pairs
   fc_load_tw_stages<N2, T>(s_tw, J.tw2);
   // The rows' lineshape coefficients (a_l, b_m, 1/F(0), fold count) staged
   // in LDS.  Read from global memory they are scalar loads, and the LDS
   // waits of the transform stages then also wait for them.
-  constexpr int ROWS_LDS = 512;
-  __shared__ float4 s_row[ROWS_LDS];
-  const bool rows_lds = M <= ROWS_LDS;
+  extern __shared__ float4 s_row[];
+  const bool rows_lds = M <= n_rows_lds;
   if (rows_lds) {
     for (int r = threadIdx.x; r < M; r += T) {
       const unsigned o = 4u * (unsigned)r;
@@ -582,8 +586,8 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
 #pragma unroll
       for (int q = 0; q < SH::RL; ++q) F[s][q] *= f0;
   };
-  if (P > 0) prefetch(0);
-  for (int p = 0; p < P; ++p) {
+  if (p1 > p0) prefetch(p0);
+  for (int p = p0; p < p1; ++p) {
     const int ra = 2 * p, rb = 2 * p + 1;
     // the pair's lineshape FTs, evaluated just before the last stage that
     // uses them (they are then not live across the earlier stages)
@@ -607,7 +611,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
       acc[s][q].y += z.y * fa[s][q] - z.x * fb[s][q];
     };
     auto next = [&] {
-      if (p + 1 < P) prefetch(p + 1);
+      if (p + 1 < p1) prefetch(p + 1);
     };
     float2* const x0 = bufs[SWAP ? (p & 1) : 0];
     float2* const x1 = bufs[SWAP ? 1 - (p & 1) : 1];
@@ -627,7 +631,7 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
     }
   }
   __syncthreads();
-  float2* Bo = J.bbuf + (long long)k1_0 * N2;
+  float2* Bo = (it.z ? J.bbuf2 : J.bbuf) + (long long)k1_0 * N2;
   // output n2 of this thread's last-stage element (s, q): j0 + s*(T/B) + q*NS, row k1 fixed
   TwProg<SH::ITL, SH::RL> tw;
   tw.init(J, k1_0 + (threadIdx.x & (B - 1)), threadIdx.x >> LB, T >> LB, NSL);
@@ -643,9 +647,10 @@ __global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict_
 
 // ---------------------------------------------------------------------------
 // Pass C with N1 fixed (see k_fft_c): first stage straight from B, last
-// stage straight to the output.
+// stage straight to the output.  SUM: B is J.bbuf + J.bbuf2 (pass B split
+// its pairs over two workgroups per row block).
 // ---------------------------------------------------------------------------
-template <int N1>
+template <int N1, bool SUM>
 __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ jobs, const int4* __restrict__ items,
                                                      int n_items) {
   constexpr int B = fc_cols_c(N1), LB = fc_log2(B), CAPE = N1 * B;
@@ -668,10 +673,17 @@ __global__ __launch_bounds__(FC_T, 4) void k_fft_c_c(const LdmJob* __restrict__ 
   }
   fc_load_tw<N1, FC_T>(s_tw, J.tw1);
   const float sc = J.out_scale;
-  auto ld = [&](int, int, int e) -> float2 {
-    return gld(J.bbuf, 8u * (unsigned)((e >> LB) * N2 + n2_0 + (e & (B - 1))));  // < 8 L <= 2^24
-  };
   auto nop = [] {};
+  auto off = [&](int e) { return 8u * (unsigned)((e >> LB) * N2 + n2_0 + (e & (B - 1))); };  // < 8 L <= 2^24
+  auto ld = [&](int, int, int e) -> float2 {
+    float2 z = gld(J.bbuf, off(e));
+    if constexpr (SUM) {
+      const float2 w = gld(J.bbuf2, off(e));
+      z.x += w.x;
+      z.y += w.y;
+    }
+    return z;
+  };
   if (J.derived) {  // uniform per workgroup: the plain store path stays as lean as without the epilogue
     auto st = [&](int, int, int e, float2 z) {
       const int n = n2_0 + (e & (B - 1)) + N2 * (e >> LB);

@@ -50,6 +50,8 @@ static_assert(LDM_AX_CHUNK == LDM_SKY_BLOCK, "a partial block is one scan chunk"
 #define FC_T 512
 #define FC_CAP_B 2048  // complex elements of a pass B tile
 #define FC_T_B 256
+#define FC_C_SUM (1 << 20)  // pass C launch-class flag: B is the sum of two pass B shares
+#define FC_B_ROWS_LDS 512  // rows whose lineshape coefficients pass B stages in LDS (more: read from global memory)
 constexpr int fc_pow2_floor(int n) { return n < 2 ? 1 : 2 * fc_pow2_floor(n >> 1); }
 // columns per workgroup of pass A and of pass C for N1, and rows per
 // workgroup of pass B for N2
@@ -187,6 +189,7 @@ struct LdmJob {
   float* ldm;                    // [ceil(m_cap/2)][ldm_stride] float2: rows 2p, 2p+1 interleaved
   float2* ybuf;                  // [(m_cap+1)/2][L]
   float2* bbuf;                  // [L]
+  float2* bbuf2;                 // [L] pass B's second share of the pairs (null: not split); pass C adds it
   float* out;                    // [nv]
   // derived outputs (null = not wanted): absorbance, transmittance, emissivity, radiance
   float *d_abs, *d_trans, *d_emis, *d_rad;

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -124,9 +125,10 @@ struct Req {
   int64_t stride = 0;
   int scale_log2 = 0;
   int bA = 0, bB = 0, bC = 0;  // log2 of the pass A / C column blocks and pass B row blocks
+  int split_b = 1;     // workgroups per pass B row block, each taking a share of the pairs
   int n_axp = 0;       // axis partials (items of k_axis)
   bool sky_whole = false;  // width extremes from the whole list's Pareto sets in k_finalize (no k_axis items)
-  size_t off_ldm = 0, off_y = 0, off_b = 0, off_rows = 0, off_axp = 0, off_wrap = 0;
+  size_t off_ldm = 0, off_y = 0, off_b = 0, off_b2 = 0, off_rows = 0, off_axp = 0, off_wrap = 0;
   const float2 *tw1 = nullptr, *tw2 = nullptr, *twl_hi = nullptr, *twl_lo = nullptr, *tw_wrap = nullptr;
 };
 
@@ -724,6 +726,7 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.ldm = (float*)(work + R.off_ldm);
   J.ybuf = (float2*)(work + R.off_y);
   J.bbuf = (float2*)(work + R.off_b);
+  J.bbuf2 = R.split_b > 1 ? (float2*)(work + R.off_b2) : nullptr;
   J.out = out;
   J.d_abs = R.derived.absorbance;
   J.d_trans = R.derived.transmittance;
@@ -756,7 +759,31 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.status = 0;
 }
 
-// Allocate workspace, twiddle tables and work-item lists for every request.
+// Workgroups per pass B row block (Req::split_b) for requests reqs[0..n)
+// that run together.  A compiled pass B workgroup walks every pair of its
+// row block; when a launch has few row blocks (a 500,001-point grid at
+// L = 512 x 2048: 512 workgroups, two per CU) the device runs them at two
+// waves per SIMD, latency-bound.  Splitting the pairs over two workgroups
+// (pass C adds the two partial spectra) doubles the waves when the doubled
+// launch still fits FC_B_SPLIT_WGS_PER_CU workgroups per CU at once; it
+// costs an inverse transform per share and L*8 bytes written and read.
+// Measured on that grid: pass B 44.2 -> 37.7 us; on L = 640 x 2048 with 72
+// pairs (1280 workgroups, more than fit at once) the split was slower.
+constexpr int FC_B_SPLIT_WGS_PER_CU = 4;
+constexpr int FC_B_SPLIT_MIN_PAIRS = 4;
+void choose_split_b(Req* reqs, int n, int n_cu) {
+  std::map<int, long long> blocks;  // compiled pass B class (N2) -> row blocks of the launch
+  for (int i = 0; i < n; ++i)
+    if (reqs[i].plan.fc) blocks[reqs[i].plan.n2] += reqs[i].plan.n1 / fc_rows_b(reqs[i].plan.n2);
+  for (int i = 0; i < n; ++i) {
+    Req& R = reqs[i];
+    R.split_b = 1;
+    if (R.plan.fc && (R.m_cap + 1) / 2 >= FC_B_SPLIT_MIN_PAIRS &&
+        2 * blocks[R.plan.n2] <= (long long)FC_B_SPLIT_WGS_PER_CU * n_cu)
+      R.split_b = 2;
+  }
+}
+
 // Workspace of one request (tile width, pass splits and the offsets of its
 // buffers from `total`, which it advances); shared by the batch layout and
 // radis_ldm_request_bytes.
@@ -784,6 +811,8 @@ int layout_request(Req& R, size_t& total) {
   R.off_ldm = total; total = align_up(total + (size_t)pcap * 2 * R.stride * 4, 256);
   R.off_y = total;   total = align_up(total + (size_t)pcap * R.plan.L * 8, 256);
   R.off_b = total;   total = align_up(total + (size_t)R.plan.L * 8, 256);
+  // pass B's second share of the pairs (split pass B)
+  R.off_b2 = total;  if (R.split_b > 1) total = align_up(total + (size_t)R.plan.L * 8, 256);
   R.off_rows = total; total = align_up(total + (size_t)R.m_cap * 16, 256);
   // axis partials: at most one per chunk, plus the two partial blocks at the window edges
   R.off_axp = total; total = align_up(total + (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2) * 32, 256);
@@ -801,7 +830,8 @@ int check_layout(const Req& R, size_t total) {
   const size_t parts = (size_t)((R.i1 - R.i0 + LDM_AX_CHUNK - 1) / LDM_AX_CHUNK + 2);
   const bool ok = (size_t)R.stride >= V && R.off_ldm + (size_t)((R.m_cap + 1) / 2) * 2 * R.stride * 4 <= total &&
                   R.off_y + (size_t)((R.m_cap + 1) / 2) * R.plan.L * 8 <= total &&
-                  R.off_b + (size_t)R.plan.L * 8 <= total && R.off_rows + (size_t)R.m_cap * 16 <= total &&
+                  R.off_b + (size_t)R.plan.L * 8 <= total &&
+                  (R.split_b == 1 || R.off_b2 + (size_t)R.plan.L * 8 <= total) && R.off_rows + (size_t)R.m_cap * 16 <= total &&
                   R.off_axp + parts * 32 <= total && (size_t)R.n_axp <= parts &&
                   R.off_wrap + (size_t)R.wrap_rows * R.plan.wrap_k * 4 <= total;
   return ok ? 0 : fail(RADIS_LDM_EINVAL, "internal: request workspace layout exceeds its %zu bytes", total);
@@ -820,6 +850,7 @@ struct LdmItemPlan {
   long long hi_a, hi_e;         // lines [hi_a, hi_e) scanned in chunks
   int n_lo, n_g, n_hi;          // items of the three axis parts
   int n_wrap, n_fill, pcap, n_colA, n_rowB, n_colC;
+  int split_b;                  // pass B items per row block (n_rowB counts them all)
 };
 
 // Item i of the request (job field 0), i < plan_items(P).
@@ -848,7 +879,7 @@ __host__ __device__ inline int4 plan_item(const LdmItemPlan& P, int i) {
   if (k < P.n_wrap) return make_int4(0, k, 0, 0);                 // L-axis node pair
   if ((k -= P.n_wrap) < P.n_fill) return make_int4(0, k, 0, 0);  // fill tile
   if ((k -= P.n_fill) < P.pcap * P.n_colA) return make_int4(0, k / P.n_colA, k % P.n_colA, 0);  // (pair, column block)
-  if ((k -= P.pcap * P.n_colA) < P.n_rowB) return make_int4(0, k, 0, 0);  // pass B row block
+  if ((k -= P.pcap * P.n_colA) < P.n_rowB) return make_int4(0, k / P.split_b, k % P.split_b, 0);  // pass B (row block, share)
   return make_int4(0, 0, k - P.n_rowB, 0);                        // pass C column block
 }
 
@@ -891,7 +922,8 @@ LdmItemPlan item_plan(const radis_ldm_db& db, Req& R) {
   P.n_fill = R.n_tiles;
   P.pcap = (R.m_cap + 1) / 2;
   P.n_colA = R.plan.n2 >> R.bA;
-  P.n_rowB = R.plan.n1 >> R.bB;
+  P.split_b = R.split_b;
+  P.n_rowB = (R.plan.n1 >> R.bB) * R.split_b;
   P.n_colC = R.plan.n2 >> R.bC;
   return P;
 }

@@ -103,11 +103,14 @@ struct HostBatch {
   std::vector<LdmJob> jobs;
 };
 
-char g_work_base[1];  // job tables hold offsets from a workspace base; never dereferenced here
+char g_work_base[1];
+int g_splits = 0;  // requests laid out with a split pass B  // job tables hold offsets from a workspace base; never dereferenced here
 
-void layout(const radis_ldm_db& db, HostBatch& B) {
+void layout(const radis_ldm_db& db, HostBatch& B, int n_cu) {
   B.total = 0;
+  choose_split_b(B.reqs.data(), (int)B.reqs.size(), n_cu);
   for (Req& R : B.reqs) {
+    g_splits += R.split_b == 2;
     const int rc = layout_request(R, B.total);
     CHECK(rc == 0, "layout_request: %s", g_err);
   }
@@ -131,6 +134,10 @@ void layout(const radis_ldm_db& db, HostBatch& B) {
         CHECK(2 * t.y < R.wrap_rows, "wrap pair beyond the table");
       } else if (i < nax + P.n_wrap + P.n_fill) {
         CHECK(t.y >= 0 && t.y < R.n_tiles, "tile");
+      } else if (i >= nax + P.n_wrap + P.n_fill + P.pcap * P.n_colA && i < n - P.n_colC) {
+        // pass B: (row block, share), each once, in order
+        const int k = i - (nax + P.n_wrap + P.n_fill + P.pcap * P.n_colA);
+        CHECK(t.y == k / R.split_b && t.z == k % R.split_b && t.y < (R.plan.n1 >> R.bB), "pass B item");
       }
       t.x = (int)j;
       B.items.push_back(t);
@@ -144,6 +151,9 @@ void layout(const radis_ldm_db& db, HostBatch& B) {
     CHECK((size_t)R.stride >= V, "LDM stride below N_v + 1");
     CHECK(R.off_y + (size_t)((R.m_cap + 1) / 2) * R.plan.L * 8 <= B.total, "Y");
     CHECK(R.off_b + (size_t)R.plan.L * 8 <= B.total, "B");
+    CHECK(R.split_b == 1 || (R.split_b == 2 && R.plan.fc && R.off_b2 >= R.off_b + (size_t)R.plan.L * 8 &&
+                             R.off_b2 + (size_t)R.plan.L * 8 <= B.total),
+          "split pass B: second share's buffer");
     CHECK(R.off_rows + (size_t)R.m_cap * 16 <= B.total, "rows");
     CHECK(R.off_axp + (size_t)cap * 32 <= B.total, "axis partials");
     CHECK(R.off_wrap + (size_t)R.wrap_rows * R.plan.wrap_k * 4 <= B.total, "wrap table");
@@ -164,6 +174,7 @@ void layout(const radis_ldm_db& db, HostBatch& B) {
     const Req& R = B.reqs[j];
     CHECK(J.n_ax_part == R.n_axp && J.m_cap == R.m_cap && J.L == R.plan.L && J.nv == R.nv, "job fields");
     CHECK(J.wrap_k == R.plan.wrap_k && J.wrap_rows == R.wrap_rows && J.wrap_ls == 2 * R.nv, "job wrap fields");
+    CHECK((J.bbuf2 != nullptr) == (R.split_b == 2), "job second pass B share");
     CHECK(J.nrad1 == R.plan.nr1 && J.nrad2 == R.plan.nr2 && J.nrad1 <= LDM_MAX_RAD && J.nrad2 <= LDM_MAX_RAD,
           "radix plan");
   }
@@ -203,7 +214,9 @@ int main() {
       ++planned;
     }
     if (B.reqs.empty()) continue;
-    layout(db, B);
+    // compute units: pass B split on every eligible request, on some, or on none
+    const int n_cu = trial % 3 == 0 ? (1 << 20) : (trial % 3 == 1 ? 256 : 1);
+    layout(db, B, n_cu);
     // condition changes (set_request): new T, p, x on the same grid; the
     // rows grow by 1.5x when the bound rises above the allocation, and the
     // whole batch is laid out again (build_layout)
@@ -229,7 +242,7 @@ int main() {
       }
       if (grow) {
         ++replans;
-        layout(db, B);
+        layout(db, B, n_cu);
       } else {
         B.jobs.assign(B.reqs.size(), LdmJob{});
         for (size_t j = 0; j < B.reqs.size(); ++j) fill_job(db, B.reqs[j], g_work_base, nullptr, B.jobs[j]);
@@ -245,7 +258,9 @@ int main() {
     CHECK(p.L == 2 * nv || (p.wrap_k > 0 && p.wrap_k <= LDM_WRAP_KMAX && p.L >= wrap_floor(2 * nv)),
           "nv %lld: L %lld wrap %d", (long long)nv, (long long)p.L, p.wrap_k);
   }
-  std::printf("planned %d requests, %d refused, %d re-plans, %d failed checks\n", planned, refused, replans, g_fail);
+  CHECK(g_splits > 0, "no request took the split pass B");
+  std::printf("planned %d requests, %d refused, %d re-plans, %d split pass B layouts, %d failed checks\n", planned,
+              refused, replans, g_splits, g_fail);
   if (g_fail) return 1;
   std::printf("ok\n");
   return 0;

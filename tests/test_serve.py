@@ -46,8 +46,11 @@ def test_results_same_alone_and_in_the_mix(server_setup):
     srv = serve.SpectrumServer(dev, dbs, budget_bytes=8 << 30, max_batch=8, n_streams=3)
     arrivals = []
     for i in range(24):
-        arrivals.append((0.002 * i, "co2", a[i]))
-        arrivals.append((0.002 * i + 0.001, "h2o", b[i]))
+        # a burst of 16 requests at t = 0 (two databases: at least two
+        # batches, launched in the same scheduling round), then a stream
+        t = 0.0 if i < 8 else 0.002 * i
+        arrivals.append((t, "co2", a[i]))
+        arrivals.append((t + (0.001 if i >= 8 else 0.0), "h2o", b[i]))
     done = srv.run_stream(arrivals, timeout=120)
     assert len(done) == 48 and all(r.error is None for r in done)
     # continuous batching: batches overlapped on the device
