@@ -245,9 +245,9 @@ template <int N, int T, int S = 1>
 __device__ __forceinline__ void fc_load_tw_stages(float2* __restrict__ s_tw, const float2* __restrict__ tw) {
   if constexpr (S < fc_nstages(N)) {
     constexpr int NS = fc_ns(N, S), R = fc_radix(N, S), LNS = fc_log2(NS);
-    constexpr int O = fc_tw_off(N, S), STEP = N / (NS * R);
+    constexpr int O = fc_tw_off(N, S), STEP = N / (NS * R), SZ = fc_tw_size(N, S);
     static_assert(fc_pow2(NS), "stages after the first have power-of-two NS");
-    for (int i = threadIdx.x; i < fc_tw_size(N, S); i += T) {
+    for (int i = threadIdx.x; i < SZ; i += T) {
       const int q = (i >> LNS) + 1, k = i & (NS - 1);
       s_tw[O + i] = gld(tw, 8u * (unsigned)(q * k * STEP));
     }
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
 // coefficients go to the dynamic LDS (n_rows_lds float4, launch-sized).
 // ---------------------------------------------------------------------------
 template <int N2>
-__global__ __launch_bounds__(FC_T_B, 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
+__global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
                                                       const LdmAxes* __restrict__ axes,
                                                       const int4* __restrict__ items, int n_rows_lds) {
   constexpr int CAP = FC_CAP_B, T = FC_T_B;
