@@ -26,7 +26,7 @@ _CACHE_PATH = None
 # sizes with compile-time-sized passes (mirrors FC_N1 / FC_N2 of the library;
 # a candidate is kept only if the planned request reports compiled_fft)
 FC_N1 = (256, 320, 384, 512, 640, 768, 1024)
-FC_N2 = (256, 512, 1024, 2048)
+FC_N2 = (64, 128, 256, 512, 1024, 2048)
 TILE_WIDTHS = (16, 32, 64, 128, 256, 512)
 FILL_LDS_MAX = 60 * 1024  # bytes of FP64 tile cells per fill workgroup
 FFT_SLACK = 1.6           # compiled splits considered up to this factor above the smallest generic L

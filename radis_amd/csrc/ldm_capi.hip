@@ -479,7 +479,7 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
                 ca, it, G.rows);                                                                               \
     break;
     switch (G.cls) {
-      FC_B(256) FC_B(512) FC_B(1024) FC_B(2048)
+      FC_B(64) FC_B(128) FC_B(256) FC_B(512) FC_B(1024) FC_B(2048)
       case 0:
         launch_part(b, timed, slot, 5, f, l, k_fft_b<LDM_CAP_B, LDM_FFT_T_B>, grid, dim3(LDM_FFT_T_B), 0, s, cj, ca,
                     it);

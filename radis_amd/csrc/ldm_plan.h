@@ -106,7 +106,7 @@ struct FftPlan {
 
 // Sizes with compile-time-sized passes: n1 (passes A and C) and n2 (pass B).
 const int FC_N1[] = {256, 320, 384, 512, 640, 768, 1024};
-const int FC_N2[] = {256, 512, 1024, 2048};
+const int FC_N2[] = {64, 128, 256, 512, 1024, 2048};
 // A compiled split is preferred while its length is within this factor of
 // the smallest generic (7-smooth) length: its passes run ~3x faster.
 const double FC_SLACK = 1.35;

@@ -9,7 +9,7 @@ from oracle import ldm_oracle as oracle
 from radis_amd import engine, workloads
 
 FC_N1 = [256, 320, 384, 512, 640, 768, 1024]
-FC_N2 = [256, 512, 1024, 2048]
+FC_N2 = [64, 128, 256, 512, 1024, 2048]
 WSTEP = 0.002
 WMIN = 2000.0
 
