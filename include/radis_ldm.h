@@ -128,7 +128,8 @@ typedef struct {
  * radis/gpu/gpu.py:22-29, 133-154) instead of RADIS's CPU result at
  * L = 2*N_v (radis/lbl/broadening.py:1900-1902, 2340-2346).  Without this
  * flag a transform longer than 2*N_v has its circular images corrected
- * (k_wrap), so the output is the L = 2*N_v result whatever fft_len is; an
+ * (image-correction items in the fill launch), so the output is the L = 2*N_v
+ * result whatever fft_len is; an
  * explicit fft_len must then be 2*N_v itself or leave a gap the correction
  * can span (about 0.5 % of L). */
 #define RADIS_LDM_PERIOD_L 64

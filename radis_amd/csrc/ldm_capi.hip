@@ -247,7 +247,7 @@ int build_layout(radis_ldm_batch* b) {
   std::map<int, size_t> sub_off;
   std::map<int64_t, size_t> l_off;
   size_t twsz = 0;
-  sub_off[LDM_WRAP_LC] = twsz;  // k_wrap's transform
+  sub_off[LDM_WRAP_LC] = twsz;  // the image correction's transform
   twsz = align_up(twsz + (size_t)LDM_WRAP_LC * 8, 256);
   for (Req& R : b->reqs) {
     for (int N : {R.plan.n1, R.plan.n2})

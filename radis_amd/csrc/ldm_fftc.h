@@ -504,7 +504,7 @@ __global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const Ld
     u0[s][0] = (float)ks * J.inv_Lws;                    // q = 0, 1, ..., RL/2-1
     u0[s][1] = (float)(L / SH::RL - ks) * J.inv_Lws;     // q = RL-1, RL-2, ..., RL/2
   }
-  // bins below the circular-image cutoff (kk < wrap_k <= N1, k_wrap): with
+  // bins below the circular-image cutoff (kk < wrap_k <= N1, wrap_item): with
   // k = k1 + N1 k2 only k2 = 0 (kk = k1) and k2 = N2 - 1 (kk = N1 - k1)
   // qualify.  The first is slot (0, 0) of thread b < B (t >> LB == 0); the
   // second is slot (s, RL - 1) of the thread with t >> LB == NSL - 1.  The

@@ -26,7 +26,7 @@
 static_assert(LDM_AX_CHUNK == LDM_SKY_BLOCK, "a partial block is one scan chunk");
 #define LDM_TW_LO_BITS 10  // four-step twiddle split W_L^e = hi[e>>10] * lo[e&1023]
 #define LDM_NFOLD_MAX 100000
-// Circular-image correction (k_wrap): a transform of length L' > 2 N_v
+// Circular-image correction (wrap_item, ldm_wrap.h): a transform of length L' > 2 N_v
 // reproduces RADIS's L = 2 N_v result when the lineshape FT of every LDM row
 // gains, at its lowest K bins, the DFT (length L') of the difference of the
 // L- and L'-periodised Lorentzian over offsets |d| <= N_v, continued
@@ -203,7 +203,7 @@ struct LdmJob {
   const float2* tw2;             // exp(-2 pi i e/n2), e < n2
   const float2* twl_hi;          // W_L^(h*1024)
   const float2* twl_lo;          // W_L^(e), e < 1024
-  const float2* tw_wrap;         // exp(-2 pi i e/LDM_WRAP_LC), e < LDM_WRAP_LC (k_wrap's transform)
+  const float2* tw_wrap;         // exp(-2 pi i e/LDM_WRAP_LC), e < LDM_WRAP_LC (the image correction's transform)
   unsigned long long* ax_part;   // [n_ax_part][4]: per-item width extremes (wl min, wl max, wg min, wg max) as bits
   float* wrap_h;                 // [wrap_rows][wrap_k]: image correction of the lineshape FT at bins kk < wrap_k, per L-axis node
   int wrap_k;                    // 0: the transform length is the period (L = 2 N_v, or the request asked for L's own period)

@@ -996,7 +996,7 @@ __global__ __launch_bounds__(LDM_FFT_T, 4) void k_fft_a(const LdmJob* __restrict
 // Folded, normalised FT lineshape of LDM row r (N_L = nl) at bin k (|k|
 // folded to [0, L/2]); radis/lbl/broadening.py:1895-1926.  Bins below
 // wrap_k add the circular-image correction of the row's L-axis node
-// (k_wrap) before the normalisation, as the image terms belong to the
+// (wrap_item) before the normalisation, as the image terms belong to the
 // kernel that RADIS divides by its folded value at x = 0.
 __device__ __forceinline__ float lineshape_ft(const LdmJob& J, int r, int nl, long long kk) {
   const float a = J.row_a[r], bco = J.row_b[r];

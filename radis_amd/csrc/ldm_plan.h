@@ -416,7 +416,8 @@ int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int
 // Plan the transform of a spectrum of period need = 2 N_v.  With `period`
 // (RADIS_LDM_PERIOD_L) the transform length is the spectrum's period, any
 // L >= need; otherwise L is need itself or at least wrap_floor(need), and
-// k_wrap corrects the images (the result is RADIS's at L = 2 N_v).
+// the image-correction items (wrap_item) correct the images (the result is
+// RADIS's at L = 2 N_v).
 int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow_fc = true, bool period = false) {
   const int64_t need = 2 * nv;
   if (want > 0 && want < need) return fail(RADIS_LDM_EINVAL, "fft_len %lld < 2*N_v = %lld", (long long)want, (long long)need);

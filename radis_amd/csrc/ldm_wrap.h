@@ -1,4 +1,4 @@
-// k_wrap: the circular-image correction that makes a transform of length
+// wrap_item: the circular-image correction that makes a transform of length
 // L' > 2 N_v return RADIS's spectrum at L = 2 N_v (radis/lbl/broadening.py:
 // 1900-1902 rfftfreq(2*len(w)), 2340-2346 the LDM of 2*len(w) rows).
 //
