@@ -17,13 +17,21 @@
 constexpr int fc_odd(int n) { return n % 3 == 0 ? 3 : (n % 5 == 0 ? 5 : 1); }
 constexpr int fc_log2(int n) { return n <= 1 ? 0 : 1 + fc_log2(n >> 1); }
 constexpr bool fc_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// Plan variant pv = 1 at N = 256 (pass B): radices 4, 16, 4 instead of
+// 16, 16.  Pass B keeps its first and last stages' elements in registers
+// (the next pair's prefetch; the accumulated spectrum and the lineshape
+// FTs), so a radix-16 end stage over 8 rows of 256 held 16 elements per
+// thread on half the threads (187 VGPRs); radix-4 ends hold 2 x 4 on all.
+constexpr bool fc_pv_ends4(int n, int pv) { return pv == 1 && n == 256; }
 // number of stages and radix of stage s
-constexpr int fc_nstages(int n) {
+constexpr int fc_nstages(int n, int pv = 0) {
+  if (fc_pv_ends4(n, pv)) return 3;
   const int f = fc_odd(n);
   const int a = fc_log2(n / f);
   return (f > 1 ? 1 : 0) + a / 4 + (a % 4 ? 1 : 0);
 }
-constexpr int fc_radix(int n, int s) {
+constexpr int fc_radix(int n, int s, int pv = 0) {
+  if (fc_pv_ends4(n, pv)) return s == 1 ? 16 : 4;
   const int f = fc_odd(n);
   const int a = fc_log2(n / f);
   if (f > 1) {
@@ -34,7 +42,7 @@ constexpr int fc_radix(int n, int s) {
   return 1 << (a % 4);
 }
 // product of the radices before stage s
-constexpr int fc_ns(int n, int s) { return s == 0 ? 1 : fc_ns(n, s - 1) * fc_radix(n, s - 1); }
+constexpr int fc_ns(int n, int s, int pv = 0) { return s == 0 ? 1 : fc_ns(n, s - 1, pv) * fc_radix(n, s - 1, pv); }
 // the sizes with a compiled transform
 constexpr bool fc_supported(int n) {
   const int f = fc_odd(n);
@@ -62,11 +70,13 @@ constexpr int pad_ce(int c) {
 // NS >= 64 holds only q = 1 (NS entries) and its butterflies form the
 // powers W^(q k) = W^(a k) W^(b k), a + b = q, a = q/2 (products at most
 // log2(R) deep).  fc_tw_off is a stage's offset in the concatenated table.
-constexpr bool fc_tw_compact(int n, int s) { return s > 0 && fc_ns(n, s) >= 64; }
-constexpr int fc_tw_size(int n, int s) {
-  return s == 0 ? 0 : (fc_tw_compact(n, s) ? fc_ns(n, s) : fc_ns(n, s) * (fc_radix(n, s) - 1));
+constexpr bool fc_tw_compact(int n, int s, int pv = 0) { return s > 0 && fc_ns(n, s, pv) >= 64; }
+constexpr int fc_tw_size(int n, int s, int pv = 0) {
+  return s == 0 ? 0 : (fc_tw_compact(n, s, pv) ? fc_ns(n, s, pv) : fc_ns(n, s, pv) * (fc_radix(n, s, pv) - 1));
 }
-constexpr int fc_tw_off(int n, int s) { return s <= 1 ? 0 : fc_tw_off(n, s - 1) + fc_tw_size(n, s - 1); }
+constexpr int fc_tw_off(int n, int s, int pv = 0) {
+  return s <= 1 ? 0 : fc_tw_off(n, s - 1, pv) + fc_tw_size(n, s - 1, pv);
+}
 
 // One Stockham DIT stage (see ldm_fft.h for the recursion) with compile-time
 // N, radix R, Ns, B and T.  Element e = i*B + b is element i of column b.
@@ -186,19 +196,20 @@ struct FcNone {
 // b0 for even S and b1 for odd S and reads the other one, so no stage needs
 // a barrier between its reads and its writes; b1 == b0: in place.
 template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD = 5, bool TWS = false,
-          bool PP = false, int S = 0, class LD, class ST, class AF, class BL>
+          bool PP = false, int PV = 0, int S = 0, class LD, class ST, class AF, class BL>
 __device__ __forceinline__ void fc_fft_pp(float2* b0, float2* b1, const float2* __restrict__ tw, LD& ld, ST& st,
                                           AF& after_first, BL& before_last) {
-  constexpr int NST = fc_nstages(N);
+  constexpr int NST = fc_nstages(N, PV);
   if constexpr (S < NST) {
     if constexpr (S == NST - 1) before_last();
-    constexpr int TWO = (TWS && S > 0) ? fc_tw_off(N, S) : -1;
+    constexpr int TWO = (TWS && S > 0) ? fc_tw_off(N, S, PV) : -1;
     float2* ib = (PP && (S & 1) == 0) ? b1 : b0;  // stage S reads what stage S-1 wrote
     float2* ob = (PP && (S & 1)) ? b1 : b0;
-    fc_stage_io<N, B, T, fc_radix(N, S), fc_ns(N, S), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ, PD, TWO,
-                fc_tw_compact(N, S), PP>(ib, ob, tw, ld, st);
+    fc_stage_io<N, B, T, fc_radix(N, S, PV), fc_ns(N, S, PV), FWD, IN_REG && S == 0, OUT_REG && S == NST - 1, OPQ, PD,
+                TWO, fc_tw_compact(N, S, PV), PP>(ib, ob, tw, ld, st);
     if constexpr (S == 0) after_first();
-    fc_fft_pp<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, PP, S + 1>(b0, b1, tw, ld, st, after_first, before_last);
+    fc_fft_pp<N, B, T, FWD, IN_REG, OUT_REG, OPQ, PD, TWS, PP, PV, S + 1>(b0, b1, tw, ld, st, after_first,
+                                                                          before_last);
   }
 }
 template <int N, int B, int T, bool FWD, bool IN_REG, bool OUT_REG, bool OPQ, int PD = 5, bool TWS = false, class LD,
@@ -223,10 +234,10 @@ __device__ __forceinline__ void fc_fft(float2* __restrict__ buf, const float2* _
 
 // (thread pass, leg) extents of a transform's first and last stages, for
 // callers that keep per-element values in registers
-template <int N, int B, int T>
+template <int N, int B, int T, int PV = 0>
 struct FcShape {
-  static constexpr int R0 = fc_radix(N, 0), IT0 = (N / R0 * B + T - 1) / T;
-  static constexpr int RL = fc_radix(N, fc_nstages(N) - 1), ITL = (N / RL * B + T - 1) / T;
+  static constexpr int R0 = fc_radix(N, 0, PV), IT0 = (N / R0 * B + T - 1) / T;
+  static constexpr int RL = fc_radix(N, fc_nstages(N, PV) - 1, PV), ITL = (N / RL * B + T - 1) / T;
   static constexpr int ITEMS0 = N / R0 * B, ITEMSL = N / RL * B;
 };
 
@@ -241,17 +252,17 @@ __device__ __forceinline__ void fc_load_tw(float2* __restrict__ s_tw, const floa
 // S >= 1 (radix R, NS) holds W_N^(q k N/(NS R)) for k < NS and q in
 // [1, R) (q = 1 only when fc_tw_compact) at fc_tw_off(N, S) + (q-1)*NS + k.
 // The tables add up to fewer than N entries (fc_tw_off(N, fc_nstages(N))).
-template <int N, int T, int S = 1>
+template <int N, int T, int PV = 0, int S = 1>
 __device__ __forceinline__ void fc_load_tw_stages(float2* __restrict__ s_tw, const float2* __restrict__ tw) {
-  if constexpr (S < fc_nstages(N)) {
-    constexpr int NS = fc_ns(N, S), R = fc_radix(N, S), LNS = fc_log2(NS);
-    constexpr int O = fc_tw_off(N, S), STEP = N / (NS * R), SZ = fc_tw_size(N, S);
+  if constexpr (S < fc_nstages(N, PV)) {
+    constexpr int NS = fc_ns(N, S, PV), R = fc_radix(N, S, PV), LNS = fc_log2(NS);
+    constexpr int O = fc_tw_off(N, S, PV), STEP = N / (NS * R), SZ = fc_tw_size(N, S, PV);
     static_assert(fc_pow2(NS), "stages after the first have power-of-two NS");
     for (int i = threadIdx.x; i < SZ; i += T) {
       const int q = (i >> LNS) + 1, k = i & (NS - 1);
       s_tw[O + i] = gld(tw, 8u * (unsigned)(q * k * STEP));
     }
-    fc_load_tw_stages<N, T, S + 1>(s_tw, tw);
+    fc_load_tw_stages<N, T, PV, S + 1>(s_tw, tw);
   }
 }
 
@@ -406,15 +417,16 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
 // coefficients go to the dynamic LDS (n_rows_lds float4, launch-sized).
 // ---------------------------------------------------------------------------
 template <int N2>
-__global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
+__global__ __launch_bounds__(FC_T_B, (N2 >= 1024 || N2 == 256) ? 4 : 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
                                                       const LdmAxes* __restrict__ axes,
                                                       const int4* __restrict__ items, int n_rows_lds) {
   constexpr int CAP = FC_CAP_B, T = FC_T_B;
   constexpr int B = fc_rows_b(N2), LB = fc_log2(B);
   static_assert(fc_pow2(N2), "pass B rows are powers of two");
-  using SH = FcShape<N2, B, T>;
+  constexpr int PV = N2 == 256 ? 1 : 0;  // plan variant (fc_pv_ends4)
+  using SH = FcShape<N2, B, T, PV>;
   constexpr int NR0 = N2 / SH::R0;
-  constexpr int NSL = fc_ns(N2, fc_nstages(N2) - 1);
+  constexpr int NSL = fc_ns(N2, fc_nstages(N2, PV) - 1, PV);
   // Two transform buffers, used ping-pong (stage s writes one, stage s+1
   // reads it and writes the other): no stage needs a barrier between its
   // reads and its writes, and the next pair's first stage writes a buffer
@@ -422,8 +434,8 @@ __global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const Ld
   // stages the last stage reads the first stage's buffer, so the pairs
   // alternate the two roles.
   __shared__ float2 bufs[2][LdsSize<CAP>::value];
-  constexpr bool SWAP = fc_nstages(N2) % 2 == 0;
-  __shared__ float2 s_tw[fc_tw_off(N2, fc_nstages(N2))];  // per-stage twiddle tables
+  constexpr bool SWAP = fc_nstages(N2, PV) % 2 == 0;
+  __shared__ float2 s_tw[fc_tw_off(N2, fc_nstages(N2, PV), PV)];  // per-stage twiddle tables
   const int4 it = items[blockIdx.x];
   const LdmJob& J = jobs[it.x];
   if (J.status) return;
@@ -433,7 +445,7 @@ __global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const Ld
   const int P = axes[it.x].p, M = axes[it.x].m;
   const int p_mid = J.bbuf2 ? (P + 1) / 2 : P;
   const int p0 = it.z ? p_mid : 0, p1 = it.z ? P : p_mid;  // this workgroup's pairs
-  fc_load_tw_stages<N2, T>(s_tw, J.tw2);
+  fc_load_tw_stages<N2, T, PV>(s_tw, J.tw2);
   // The rows' lineshape coefficients (a_l, b_m, 1/F(0), fold count) staged
   // in LDS.  Read from global memory they are scalar loads, and the LDS
   // waits of the transform stages then also wait for them.
@@ -615,7 +627,7 @@ __global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const Ld
     };
     float2* const x0 = bufs[SWAP ? (p & 1) : 0];
     float2* const x1 = bufs[SWAP ? 1 - (p & 1) : 1];
-    fc_fft_pp<N2, B, T, true, true, true, false, 4, true, true>(x0, x1, s_tw, ld, st, next, lineshapes);
+    fc_fft_pp<N2, B, T, true, true, true, false, 4, true, true, PV>(x0, x1, s_tw, ld, st, next, lineshapes);
   }
   __syncthreads();
   // the accumulated spectrum into LDS (bufs[1]), in the forward last stage's element order
@@ -642,7 +654,7 @@ __global__ __launch_bounds__(FC_T_B, N2 >= 1024 ? 4 : 2) void k_fft_b_c(const Ld
   FcNone none;
   auto nop = [] {};
   // first stage reads bufs[1] (ping-pong)
-  fc_fft_pp<N2, B, T, false, false, true, false, 4, true, true>(bufs[0], bufs[1], s_tw, none, st_out, nop, nop);
+  fc_fft_pp<N2, B, T, false, false, true, false, 4, true, true, PV>(bufs[0], bufs[1], s_tw, none, st_out, nop, nop);
 }
 
 // ---------------------------------------------------------------------------
