@@ -38,12 +38,24 @@ class ServedRequest:
     n_v: int = 0
     t_start: float = 0.0
     t_done: float = 0.0
-    result: np.ndarray | None = None
+    result: np.ndarray | None = None  # abscoeff: float32, or float64 when its range needs it (_unscale)
     error: str | None = None
 
     @property
     def latency(self):
         return self.t_done - self.t_arrival
+
+
+def _unscale(a, s):
+    """The stored spectrum times 2**s, out of the pinned buffer: float32, as
+    the reference's gpuApp.iterate returns it (a power-of-two scale is exact
+    there), or float64 when the scaled values leave float32's normal range."""
+    if s == 0:
+        return a.copy()
+    peak = float(np.max(np.abs(a))) if a.size else 0.0
+    if abs(s) <= 126 and (peak == 0.0 or 2.0 ** -100 < peak * 2.0 ** s < 2.0 ** 100):
+        return a * np.float32(2.0 ** s)  # exact: a power of two, results stay normal
+    return np.ldexp(a.astype(np.float64), s)
 
 
 class _Flight:
@@ -161,8 +173,7 @@ class SpectrumServer:
                 if inf.status:
                     r.error = f"device status {inf.status:#x} ({engine.describe_status(inf.status)})"
                 else:
-                    a = f.host[i].numpy().astype(np.float64)
-                    r.result = np.ldexp(a, int(inf.scale_log2)) if inf.scale_log2 else a
+                    r.result = _unscale(f.host[i].numpy(), int(inf.scale_log2))
                 r.t_done = t
                 self.done.append(r)
             f.batch.close()
