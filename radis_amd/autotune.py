@@ -196,8 +196,13 @@ def fft_candidates(nv, n_generic=4):
     return out
 
 
+def fill_pad(w):
+    """LDS row stride of a fill tile beyond its width (mirrors ldm_fill_pad)."""
+    return 3 if w >= 64 else 2
+
+
 def tile_candidates(m_cap):
-    return [w for w in TILE_WIDTHS if m_cap * (w + 2) * 8 <= FILL_LDS_MAX]
+    return [w for w in TILE_WIDTHS if m_cap * (w + fill_pad(w)) * 8 <= FILL_LDS_MAX]
 
 
 def _time(db, req, stages, reps):

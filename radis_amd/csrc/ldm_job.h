@@ -18,7 +18,12 @@
 #define LDM_CAP_B 2048     // complex elements held by a small pass B workgroup
 #define LDM_FFT_T_B 256    // threads of a small pass B workgroup
 #define LDM_FILL_T 256     // threads of a fill workgroup
-#define LDM_FILL_PAD 2     // LDS row stride of a fill tile: tile_w + LDM_FILL_PAD doubles (16-byte aligned rows)
+// LDS row stride of a fill tile: tile_w + ldm_fill_pad(tile_w) doubles.  The
+// stride sets how a wave's FP64 atomics (lanes on neighbouring L rows, G rows
+// and columns) spread over the LDS banks; measured best: + 2 for 32-point
+// tiles (dense lists: 100 M lines on 600,001 points), + 3 for wider ones
+// (+ 1, + 3, + 5, + 7 measured slower or equal in each case).
+constexpr int ldm_fill_pad(int tile_w) { return tile_w >= 64 ? 3 : 2; }
 #define LDM_AX_CHUNK 4096  // lines per axis-reduction workgroup (16 per lane)
 #define LDM_SKY_BLOCK 4096  // lines per width-candidate block
 #define LDM_SKY_GROUP_MIN 8  // fewest candidate blocks per k_axis workgroup

@@ -814,8 +814,8 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 // weights (radis/lbl/broadening.py:2305-2318) of its corners that fall in
 // the tile into the LDS tile with FP64 atomics.  The tile is written to HBM
 // once, so the LDM needs no clearing and no global atomics.  Row index
-// r = l*N_L + m; LDS row stride W+2 doubles (16-byte aligned rows, rows on
-// different banks).  The tile accumulates in FP64 with ds_add_f64: on gfx950
+// r = l*N_L + m; LDS row stride W + ldm_fill_pad(W) doubles (neighbouring
+// rows on different banks).  The tile accumulates in FP64 with ds_add_f64: on gfx950
 // LDS ds_add_f32 retires far slower than the 64-bit and integer LDS atomics
 // (measured ~3 cycles per lane per CU, which bounded this kernel), and the
 // FP64 sums are also more accurate.
@@ -830,7 +830,7 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
                                           double* acc) {
   const int M = A.m;
   const int ng = A.ng, nl = A.nl;
-  const int W = J.tile_w, Wp = W + LDM_FILL_PAD;
+  const int W = J.tile_w, Wp = W + ldm_fill_pad(W);
   const long long k_lo = (long long)it.y * W;
   long long k_hi = k_lo + W;
   if (k_hi > J.v) k_hi = J.v;
@@ -857,9 +857,11 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     cols.load<NONEQ, PLAIN>(l1 < hlast ? l1 : hlast, n2);
   }
   {
-    const int n2 = (M * Wp) >> 1;  // Wp is even: whole 16-byte pairs
+    // whole 16-byte pairs, and the last cell alone when M * Wp is odd
+    const int n = M * Wp, n2 = n >> 1;
     double2* a2 = reinterpret_cast<double2*>(acc);
     for (int e = threadIdx.x; e < n2; e += blockDim.x) a2[e] = make_double2(0.0, 0.0);
+    if ((n & 1) && threadIdx.x == 0) acc[n - 1] = 0.0;
   }
   __syncthreads();
   const int wlen = (int)(k_hi - k_lo);
@@ -915,8 +917,8 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   // flush: the LDM interleaves rows 2p and 2p+1 (element n of row 2p+h at
   // float 2n+h of pair p).  A lane writes 2 columns of one pair (16 bytes)
   // per step: (2p, c), (2p+1, c), (2p, c+1), (2p+1, c+1); the second row of
-  // an odd last pair is written as zeros.  W is a power of two and LDS rows
-  // are 16-byte aligned.
+  // an odd last pair is written as zeros.  W is a power of two; even rows
+  // start 16-byte aligned.
   const int P2 = (M + 1) >> 1;
   const int W2 = W >> 1, sh2 = __builtin_ctz(W2);
   const long long pstride = 2 * J.ldm_stride;  // floats per pair
@@ -926,7 +928,10 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     if (c >= wlen) continue;
     const double* src = acc + (2 * p) * Wp + c;
     const double2 a = *reinterpret_cast<const double2*>(src);
-    const double2 b = (2 * p + 1 < M) ? *reinterpret_cast<const double2*>(src + Wp) : make_double2(0.0, 0.0);
+    // row 2p starts on an even cell (16-byte aligned), row 2p+1 on an odd one when Wp is odd
+    const double2 b = (2 * p + 1 < M) ? ((Wp & 1) ? make_double2(src[Wp], src[Wp + 1])
+                                                   : *reinterpret_cast<const double2*>(src + Wp))
+                                      : make_double2(0.0, 0.0);
     float* dst = out + p * pstride + 2 * c;
     if (c + 2 <= wlen) *reinterpret_cast<float4*>(dst) = make_float4((float)a.x, (float)b.x, (float)a.y, (float)b.y);
     else *reinterpret_cast<float2*>(dst) = make_float2((float)a.x, (float)b.x);

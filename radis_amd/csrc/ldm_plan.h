@@ -626,7 +626,7 @@ int bound_request(const radis_ldm_db& db, Req& R) {
   return 0;
 }
 
-size_t fill_tile_bytes(int m_cap, int w) { return (size_t)m_cap * (size_t)(w + LDM_FILL_PAD) * sizeof(double); }
+size_t fill_tile_bytes(int m_cap, int w) { return (size_t)m_cap * (size_t)(w + ldm_fill_pad(w)) * sizeof(double); }
 
 int choose_tile(int m_cap) {
   int w = MAX_TILE_W;

@@ -29,7 +29,7 @@ def test_fft_candidates_cover_the_grid(nv):
 def test_tile_candidates_fit_lds():
     for m_cap in (4, 27, 40, 144, 300):
         ws = autotune.tile_candidates(m_cap)
-        assert ws and all(m_cap * (w + 2) * 8 <= autotune.FILL_LDS_MAX for w in ws)
+        assert ws and all(m_cap * (w + autotune.fill_pad(w)) * 8 <= autotune.FILL_LDS_MAX for w in ws)
     assert autotune.tile_candidates(500) == []
 
 
