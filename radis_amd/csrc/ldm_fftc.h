@@ -8,7 +8,8 @@
 // W_N^e (e < N) staged in LDS once per workgroup.
 //
 // Plan for N = 2^a * f, f in {1, 3, 5}: the odd factor f first (no twiddles),
-// then radix-16 stages, then one radix-2/4/8 stage for the remaining power.
+// then radix-16 stages, then one radix-2/4/8 stage for the remaining power
+// (pass B at N = 256 runs a variant with radix-4 end stages, fc_pv_ends4).
 // Columns: B = 2^bs independent columns of length N, element i of column b at
 // LDS slot pad_idx(i*B + b) (the same layout as the generic transform).
 #pragma once
