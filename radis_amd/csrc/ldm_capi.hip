@@ -201,6 +201,7 @@ struct radis_ldm_batch {
   std::vector<Group> grp_a, grp_b, grp_c;
   bool tiles_valid = false;  // fill tile bounds computed for the current layout
   size_t fill_lds = 0;
+  bool fill_grouped = false;  // some request fills its tiles in row groups
   std::vector<float*> outs;        // outputs of the last eager run
   std::vector<float*> graph_outs;  // outputs bound to the captured graph
   hipEvent_t ev_done[2] = {nullptr, nullptr};
@@ -237,11 +238,13 @@ int build_layout(radis_ldm_batch* b) {
   // --- workspace
   size_t total = 0;
   size_t fill_lds = 0;
+  bool fill_grouped = false;
   choose_split_b(b->reqs.data(), (int)b->reqs.size(), b->ctx->n_cu);
   for (Req& R : b->reqs) {
     int rc = layout_request(R, total);
     if (rc) return rc;
-    fill_lds = std::max(fill_lds, fill_tile_bytes(R.m_cap, R.tile_w));
+    fill_lds = std::max(fill_lds, fill_tile_bytes(R.fill_rows, R.tile_w));
+    fill_grouped = fill_grouped || R.fill_groups > 1;
   }
   // --- twiddle tables, deduplicated by size
   std::map<int, size_t> sub_off;
@@ -352,6 +355,7 @@ int build_layout(radis_ldm_batch* b) {
   // (and `all` outlives the copy)
   HIPCHK(hipStreamSynchronize(b->ctx->side));
   b->fill_lds = std::max<size_t>(fill_lds, 4);
+  b->fill_grouped = fill_grouped;
   return 0;
 }
 
@@ -421,6 +425,7 @@ struct LaunchSet {
   const LaunchGroup *ga, *gb, *gc;
   int nga, ngb, ngc;
   size_t fill_lds;
+  bool fill_grouped;  // the fill kernel with row groups (some request's LDM is taller than a tile)
   bool tile_bounds;  // run k_tile_bounds (else the items hold valid tile line ranges)
   int n_cu;
 };
@@ -446,10 +451,17 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
   if (X.n_wrap + X.n_fill) {
     const size_t lds = X.n_wrap ? std::max(X.fill_lds, (size_t)LDM_WRAP_LDS * sizeof(float2)) : X.fill_lds;
     const dim3 grid(X.n_wrap + X.n_fill);
-    if (!dbd.iso && !dbd.delta && !dbd.n_self)
-      launch(b, timed, slot, 3, k_fill<true>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
-    else
-      launch(b, timed, slot, 3, k_fill<false>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
+    const bool plain = !dbd.iso && !dbd.delta && !dbd.n_self;
+    if (X.fill_grouped) {
+      if (plain)
+        launch(b, timed, slot, 3, k_fill<true, true>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
+      else
+        launch(b, timed, slot, 3, k_fill<false, true>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
+    } else if (plain) {
+      launch(b, timed, slot, 3, k_fill<true, false>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
+    } else {
+      launch(b, timed, slot, 3, k_fill<false, false>, grid, dim3(LDM_FILL_T), lds, s, cj, ca, wrap_fill, X.n_wrap, dbd);
+    }
   }
   items += X.n_fill;
   for (int g = 0; g < X.nga; ++g) {
@@ -533,6 +545,7 @@ int launch_kernels(radis_ldm_batch* b, hipStream_t s, bool timed, int slot, bool
   X.gc = b->grp_c.data();
   X.ngc = (int)b->grp_c.size();
   X.fill_lds = b->fill_lds;
+  X.fill_grouped = b->fill_grouped;
   // Tile line bounds depend on the grid and, through the pressure-shift
   // margin, on p; without a shift column they are fixed per layout.
   X.tile_bounds = db.has_delta || !b->tiles_valid;
@@ -806,7 +819,7 @@ int radis_ldm_request_bytes(const radis_ldm_db* db, const radis_ldm_request* req
   rc = layout_request(R, total);
   if (rc) return rc;
   // the batch's work items and job table (16 B per item, a few per tile and block)
-  int64_t b = (int64_t)total + (int64_t)R.n_tiles * 16;
+  int64_t b = (int64_t)total + (int64_t)R.n_tiles * R.fill_groups * 16;
   // twiddle tables, plus alignment slack: an upper bound of what a batch of
   // this one request allocates
   b += (int64_t)(R.plan.n1 + R.plan.n2 + 1024 + (R.plan.L >> LDM_TW_LO_BITS) + 1) * 8 + 64 * 1024;
@@ -1169,7 +1182,8 @@ int radis_ldm_run_noalloc(radis_ldm_ctx* ctx, const radis_ldm_db* db, const radi
   X.ngb = 1;
   X.gc = &gc;
   X.ngc = 1;
-  X.fill_lds = std::max<size_t>(fill_tile_bytes(R.m_cap, R.tile_w), 4);
+  X.fill_lds = std::max<size_t>(fill_tile_bytes(R.fill_rows, R.tile_w), 4);
+  X.fill_grouped = R.fill_groups > 1;
   X.tile_bounds = true;
   X.n_cu = ctx->n_cu;
   rc = launch_set(X, db->dev, nullptr, s, false, 0);

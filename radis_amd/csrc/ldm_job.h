@@ -190,6 +190,7 @@ struct LdmJob {
   int rad1[LDM_MAX_RAD], rad2[LDM_MAX_RAD];
   int bA, bB;                    // columns per workgroup in passes A/C and pass B
   int m_cap, tile_w, n_tiles;
+  int fill_rows, fill_groups;    // LDM rows per fill row group, groups per tile (1: the tile holds every row)
   long long ldm_stride;          // elements per LDM row
   float* ldm;                    // [ceil(m_cap/2)][ldm_stride] float2: rows 2p, 2p+1 interleaved
   float2* ybuf;                  // [(m_cap+1)/2][L]

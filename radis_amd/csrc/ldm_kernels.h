@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, 
   if (g >= n_items) return;
   const int4 it = items[g];
   LdmJob& J = jobs[it.x];
-  const long long t = it.y;
+  const long long t = it.y / J.fill_groups;  // item y = tile * fill_groups + row group
   const long long k_lo = t * J.tile_w;
   long long k_hi = k_lo + J.tile_w;
   if (k_hi > J.v) k_hi = J.v;
@@ -560,12 +560,17 @@ __global__ __launch_bounds__(256) void k_tile_bounds(LdmJob* __restrict__ jobs, 
 // I = S(T) * number density * 2^-scale in FP32.
 //
 // The grid index and its fraction are always FP64 (np.interp rule).  The two
-// log-width positions are first evaluated in FP32 (relative error of the
-// widths below 2e-6, i.e. < 2e-5 in axis units); only when a position lies
-// within LDM_TIE_EPS of an axis node, or J.fp64_widths is set, are they
+// log-width positions are first evaluated in FP32 from hardware base-2
+// logarithms: log2 w within LDM_TIE_LOG2 (a few ulp of |log2 w| <= 32: the
+// logarithms, the sum with the isotopologue term, the FP32 axis origin),
+// i.e. within LDM_TIE_LOG2 / dx2 axis steps for an axis step of dx2 in log2
+// units.  Only when a position lies within the margin tie = max(LDM_TIE_EPS,
+// LDM_TIE_LOG2 / dx2) of an axis node, or J.fp64_widths is set, are they
 // recomputed in FP64 with NumPy's rule, so the integer indices always equal
-// the FP64 ones.
+// the FP64 ones.  (RADIS's default axes, dx2 ~ 0.2 and 0.3, keep the margin
+// at LDM_TIE_EPS; fine axes, dxG = 0.005, widen it to ~2e-3 of a step.)
 #define LDM_TIE_EPS 2e-4f
+#define LDM_TIE_LOG2 1.5e-5f
 
 // s * exp(x) rounded to FP32 for |x| up to ~700: the exponent x (FP64) is
 // reduced in FP64, x = n ln2 + r with |r| <= ln2/2, so the FP32 exp of r
@@ -676,6 +681,7 @@ struct LineConst {
   double c_boltz, i_scale;
   float c_stim_T, c_stim_R, log_r_f, pa_air_f, pa_self_f;
   float laG0_f, inv_dxG_f, laL0_f, inv_dxL_f;
+  float tie_g, tie_l;  // tie margins of the FP32 axis positions (axis steps)
   int nv, fp64_widths, has_delta, min_rms, i_fast;
   double c_vib, c_rot, c_ref;
   const LdmJob* job;
@@ -685,7 +691,9 @@ struct LineConst {
         inv_delta_f(J.inv_delta_f), p_atm(J.p_atm), x_air(J.x_air),
         x_self(J.x_self), log_r(J.log_r), c_boltz(J.c_boltz), i_scale(J.i_scale), c_stim_T(J.c_stim_T),
         c_stim_R(J.c_stim_R), log_r_f(J.log_r_f), pa_air_f(J.pa_air_f), pa_self_f(J.pa_self_f),
-        laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f), nv((int)J.nv),
+        laG0_f(J.laG0_f), inv_dxG_f(J.inv_dxG_f), laL0_f(J.laL0_f), inv_dxL_f(J.inv_dxL_f),
+        tie_g(fmaxf(LDM_TIE_EPS, LDM_TIE_LOG2 * J.inv_dxG_f)), tie_l(fmaxf(LDM_TIE_EPS, LDM_TIE_LOG2 * J.inv_dxL_f)),
+        nv((int)J.nv),
         fp64_widths(J.fp64_widths), has_delta(db.delta != nullptr), min_rms(J.min_rms), i_fast(J.i_fast), c_vib(J.c_vib),
         c_rot(J.c_rot), c_ref(J.c_ref), job(&J), axes(&A) {}
 };
@@ -712,9 +720,9 @@ __device__ __forceinline__ void min_rms_weights(const LdmJob& J, float log_wg, f
   t.tl = al;
 }
 
-__device__ __forceinline__ bool near_node(float p, int n) {
+__device__ __forceinline__ bool near_node(float p, int n, float eps) {
   const float f = p - floorf(p);
-  return f < LDM_TIE_EPS || f > 1.0f - LDM_TIE_EPS || p < LDM_TIE_EPS || p > (float)(n - 1) - LDM_TIE_EPS;
+  return f < eps || f > 1.0f - eps || p < eps || p > (float)(n - 1) - eps;
 }
 
 // FP64 Lorentzian FWHM from the request constants (same expression as
@@ -754,8 +762,8 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
   const float l2wl = __builtin_amdgcn_logf(wl_f);          // log2 FWHM_L
   const float pg = (l2wg - C.laG0_f) * C.inv_dxG_f;
   const float pl = (l2wl - C.laL0_f) * C.inv_dxL_f;
-  const bool exact_g = C.fp64_widths || near_node(pg, ng);
-  const bool exact_l = C.fp64_widths || near_node(pl, nl);
+  const bool exact_g = C.fp64_widths || near_node(pg, ng, C.tie_g);
+  const bool exact_l = C.fp64_widths || near_node(pl, nl, C.tie_l);
   if (exact_g) {
     // the exact path is rare (positions within 2e-4 of a node): its tables
     // stay in global memory, where they cost the fill no LDS
@@ -825,13 +833,22 @@ __device__ __forceinline__ void line_terms(const LineConst& C, const LineRaw& r,
 // sits on the common path (only atomics), so atomics retire in the background.
 // ---------------------------------------------------------------------------
 
-template <bool NONEQ, bool PLAIN>
+// GROUPED: the LDM is taller than one tile's LDS (J.fill_groups > 1 for some
+// request of the launch); item y = tile * fill_groups + g, and the workgroup
+// keeps rows [g * fill_rows, g * fill_rows + MR) only: each of its 8 atomics
+// checks its corner's row.  A request with fill_groups = 1 has one group of
+// every row.
+template <bool NONEQ, bool PLAIN, bool GROUPED>
 __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, const int4 it, const LdmDb& db,
                                           double* acc) {
   const int M = A.m;
   const int ng = A.ng, nl = A.nl;
   const int W = J.tile_w, Wp = W + ldm_fill_pad(W);
-  const long long k_lo = (long long)it.y * W;
+  const int tile = GROUPED ? it.y / J.fill_groups : it.y;
+  const int r0 = GROUPED ? (it.y - tile * J.fill_groups) * J.fill_rows : 0;  // even
+  const int MR = GROUPED ? min(J.fill_rows, M - r0) : M;                     // rows of this group
+  if (GROUPED && MR <= 0) return;  // uniform: before any barrier
+  const long long k_lo = (long long)tile * W;
   long long k_hi = k_lo + W;
   if (k_hi > J.v) k_hi = J.v;
   const LineConst C(J, A, db);
@@ -857,8 +874,8 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     cols.load<NONEQ, PLAIN>(l1 < hlast ? l1 : hlast, n2);
   }
   {
-    // whole 16-byte pairs, and the last cell alone when M * Wp is odd
-    const int n = M * Wp, n2 = n >> 1;
+    // whole 16-byte pairs, and the last cell alone when MR * Wp is odd
+    const int n = MR * Wp, n2 = n >> 1;
     double2* a2 = reinterpret_cast<double2*>(acc);
     for (int e = threadIdx.x; e < n2; e += blockDim.x) a2[e] = make_double2(0.0, 0.0);
     if ((n & 1) && threadIdx.x == 0) acc[n - 1] = 0.0;
@@ -890,7 +907,18 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     if (c0 < -1 || c0 > wlen - 1) continue;
     // the stencil's cells are base + {0, 1} + {0, nl*Wp} + {0, Wp}; the
     // weight of cell (c, l, m) is fc[c] * (fg[l] * fl[m])
-    double* base = acc + (t.l0 * nl + t.m0) * Wp + c0 + x_off;
+    const int rb = t.l0 * nl + t.m0 - r0;  // the stencil's first row in the group
+    double* base = acc + rb * Wp + c0 + x_off;
+    // GROUPED: which of this lane's corner rows (its first, + L step, + G
+    // step, both) fall in the group
+    bool q00 = true, q01 = true, q10 = true, q11 = true;
+    if constexpr (GROUPED) {
+      const int ra = rb + (x1 ? nl : 0) + (x2 ? 1 : 0), sl = x2 ? -1 : 1, sgr = x1 ? -nl : nl;
+      q00 = (unsigned)ra < (unsigned)MR;
+      q01 = (unsigned)(ra + sl) < (unsigned)MR;
+      q10 = (unsigned)(ra + sgr) < (unsigned)MR;
+      q11 = (unsigned)(ra + sgr + sl) < (unsigned)MR;
+    }
     const float ug = 1.0f - t.tg, ul = 1.0f - t.tl;
     const float fg0 = x1 ? t.tg : ug, fg1 = x1 ? ug : t.tg;  // this lane's corner dl = 0, 1
     const float fl0 = x2 ? t.tl : ul, fl1 = x2 ? ul : t.tl;
@@ -899,18 +927,18 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
     // columns c0 (needs c0 >= 0) and c0 + 1 (needs c0 + 1 < wlen)
     if (x0 ? (c0 + 1 < wlen) : (c0 >= 0)) {
       const float v = x0 ? v1 : v0;
-      atomicAdd(base, (double)(v * w00));
-      atomicAdd(base + s_l, (double)(v * w01));
-      atomicAdd(base + s_g, (double)(v * w10));
-      atomicAdd(base + s_g + s_l, (double)(v * w11));
+      if (q00) atomicAdd(base, (double)(v * w00));
+      if (q01) atomicAdd(base + s_l, (double)(v * w01));
+      if (q10) atomicAdd(base + s_g, (double)(v * w10));
+      if (q11) atomicAdd(base + s_g + s_l, (double)(v * w11));
     }
     if (x0 ? (c0 >= 0) : (c0 + 1 < wlen)) {
       const float v = x0 ? v0 : v1;
       double* b1 = base + s_c;
-      atomicAdd(b1, (double)(v * w00));
-      atomicAdd(b1 + s_l, (double)(v * w01));
-      atomicAdd(b1 + s_g, (double)(v * w10));
-      atomicAdd(b1 + s_g + s_l, (double)(v * w11));
+      if (q00) atomicAdd(b1, (double)(v * w00));
+      if (q01) atomicAdd(b1 + s_l, (double)(v * w01));
+      if (q10) atomicAdd(b1 + s_g, (double)(v * w10));
+      if (q11) atomicAdd(b1 + s_g + s_l, (double)(v * w11));
     }
   }
   __syncthreads();
@@ -919,17 +947,17 @@ __device__ __forceinline__ void fill_tile(const LdmJob& J, const LdmAxes& A, con
   // per step: (2p, c), (2p+1, c), (2p, c+1), (2p+1, c+1); the second row of
   // an odd last pair is written as zeros.  W is a power of two; even rows
   // start 16-byte aligned.
-  const int P2 = (M + 1) >> 1;
+  const int P2 = (MR + 1) >> 1;
   const int W2 = W >> 1, sh2 = __builtin_ctz(W2);
-  const long long pstride = 2 * J.ldm_stride;  // floats per pair
-  float* const out = J.ldm + 2 * k_lo;          // column k_lo of pair 0
+  const long long pstride = 2 * J.ldm_stride;                   // floats per pair
+  float* const out = J.ldm + 2 * k_lo + (long long)(r0 >> 1) * pstride;  // column k_lo of the group's first pair
   for (int w = threadIdx.x; w < (P2 << sh2); w += LDM_FILL_T) {
     const int p = w >> sh2, c = 2 * (w & (W2 - 1));
     if (c >= wlen) continue;
     const double* src = acc + (2 * p) * Wp + c;
     const double2 a = *reinterpret_cast<const double2*>(src);
     // row 2p starts on an even cell (16-byte aligned), row 2p+1 on an odd one when Wp is odd
-    const double2 b = (2 * p + 1 < M) ? ((Wp & 1) ? make_double2(src[Wp], src[Wp + 1])
+    const double2 b = (2 * p + 1 < MR) ? ((Wp & 1) ? make_double2(src[Wp], src[Wp + 1])
                                                    : *reinterpret_cast<const double2*>(src + Wp))
                                       : make_double2(0.0, 0.0);
     float* dst = out + p * pstride + 2 * c;

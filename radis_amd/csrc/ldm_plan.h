@@ -36,6 +36,7 @@ const int STAT_BLOCK = 4096;             // lines per host-side statistics block
 const int FILL_LDS_TARGET = 44 * 1024;   // preferred LDS bytes of one fill tile (FP64 cells; 3 tiles per CU)
 const int FILL_LDS_MAX = 60 * 1024;      // hard limit (plus ~2.2 KB of static axis tables, < 64 KB)
 const int MIN_TILE_W = 16, MAX_TILE_W = 4096;
+const int FILL_GROUP_TILE_W = 32;      // tile width of fills in row groups
 const int N1_MAX = 1024, N2_MAX = 8192;
 
 // last error of the calling thread (a fixed buffer: reporting an error
@@ -122,6 +123,7 @@ struct Req {
   int m_cap = 0;       // allocated rows
   int wrap_rows = 0;   // L-axis nodes of the image-correction table (N_L <= M/2: N_G >= 2)
   int tile_w = 0, n_tiles = 0;
+  int fill_rows = 0, fill_groups = 1;  // row groups of a fill tile (an LDM taller than one tile's LDS)
   int64_t stride = 0;
   int scale_log2 = 0;
   int bA = 0, bB = 0, bC = 0;  // log2 of the pass A / C column blocks and pass B row blocks
@@ -723,6 +725,8 @@ void fill_job(const radis_ldm_db& db, const Req& R, char* work, float* out, LdmJ
   J.m_cap = R.m_cap;
   J.tile_w = R.tile_w;
   J.n_tiles = R.n_tiles;
+  J.fill_rows = R.fill_rows;
+  J.fill_groups = R.fill_groups;
   J.ldm_stride = R.stride;
   J.ldm = (float*)(work + R.off_ldm);
   J.ybuf = (float2*)(work + R.off_y);
@@ -790,9 +794,20 @@ void choose_split_b(Req* reqs, int n, int n_cu) {
 // radis_ldm_request_bytes.
 int layout_request(Req& R, size_t& total) {
   R.tile_w = R.r.tile_w ? R.r.tile_w : choose_tile(R.m_cap);
-  if (fill_tile_bytes(R.m_cap, R.tile_w) > (size_t)FILL_LDS_MAX)
-    return fail(RADIS_LDM_ERANGE, "LDM of %d rows does not fit a fill tile of width %d (max %d rows at width %d)",
-                R.m_cap, R.tile_w, (int)(FILL_LDS_MAX / fill_tile_bytes(1, MIN_TILE_W)), MIN_TILE_W);
+  R.fill_rows = R.m_cap;
+  R.fill_groups = 1;
+  if (fill_tile_bytes(R.m_cap, R.tile_w) > (size_t)FILL_LDS_MAX) {
+    // An LDM taller than one tile's LDS: every tile is filled in row groups
+    // of an even number of rows (whole interleaved pairs), each group its own
+    // workgroup reading the tile's lines again and keeping the corners that
+    // fall in its rows.  32-point tiles (128-B flush segments per pair).
+    if (!R.r.tile_w) R.tile_w = FILL_GROUP_TILE_W;
+    const int rows = (int)((size_t)FILL_LDS_TARGET / fill_tile_bytes(1, R.tile_w)) & ~1;
+    if (rows < 2)
+      return fail(RADIS_LDM_ERANGE, "fill tile of width %d does not fit two LDM rows", R.tile_w);
+    R.fill_rows = rows;
+    R.fill_groups = (R.m_cap + rows - 1) / rows;
+  }
   const int64_t V = R.nv + 1;
   R.n_tiles = (int)((V + R.tile_w - 1) / R.tile_w);
   R.stride = (int64_t)align_up((size_t)V, 64);
@@ -920,7 +935,7 @@ LdmItemPlan item_plan(const radis_ldm_db& db, Req& R) {
   }
   R.n_axp = P.n_lo + P.n_g + P.n_hi;
   P.n_wrap = (R.wrap_rows + 1) / 2;
-  P.n_fill = R.n_tiles;
+  P.n_fill = R.n_tiles * R.fill_groups;  // item y = tile * fill_groups + group
   P.pcap = (R.m_cap + 1) / 2;
   P.n_colA = R.plan.n2 >> R.bA;
   P.split_b = R.split_b;

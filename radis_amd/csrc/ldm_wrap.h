@@ -131,9 +131,10 @@ __device__ __forceinline__ void wrap_item(const LdmJob* __restrict__ jobs, const
 // (items[0, n_wrap)), the others the fill tiles that follow them in the
 // item list.  One kernel per line-list variant (PLAIN: no isotopologue,
 // shift or n_self column; the host picks), so each gets its own register
-// allocation: the plain kernel needs the fewest.  The dynamic LDS holds the
-// larger of a fill tile and an image-correction item.
-template <bool PLAIN>
+// allocation: the plain kernel needs the fewest; GROUPED (fill_tile) only
+// when some request's LDM is taller than a tile.  The dynamic LDS holds the
+// larger of a fill tile (or row group) and an image-correction item.
+template <bool PLAIN, bool GROUPED>
 __global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict__ jobs,
                                                      const LdmAxes* __restrict__ axes,
                                                      const int4* __restrict__ items, int n_wrap, LdmDb db) {
@@ -146,6 +147,6 @@ __global__ __launch_bounds__(LDM_FILL_T, 4) void k_fill(const LdmJob* __restrict
   const LdmJob& J = jobs[it.x];
   const LdmAxes& A = axes[it.x];
   if (A.m == 0 || J.status) return;
-  if (J.noneq) fill_tile<true, PLAIN>(J, A, it, db, acc);
-  else fill_tile<false, PLAIN>(J, A, it, db, acc);
+  if (J.noneq) fill_tile<true, PLAIN, GROUPED>(J, A, it, db, acc);
+  else fill_tile<false, PLAIN, GROUPED>(J, A, it, db, acc);
 }

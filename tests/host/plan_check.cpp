@@ -86,6 +86,10 @@ radis_ldm_request make_request(std::mt19937_64& rng, bool whole) {
   r.Tref = 296.0;
   r.dxG = std::log(std::sqrt(0.01 / 0.46) + 1);
   r.dxL = std::log(std::sqrt(0.01 / 0.20) + 1);
+  if (rng() % 4 == 0) {  // fine axes: tall LDMs, filled in row groups
+    r.dxG = 0.01;
+    r.dxL = 0.02;
+  }
   r.folding_thresh = 1e-6;
   for (int s = 0; s < RADIS_LDM_MAX_ISO; ++s) r.q_ratio[s] = 0.5 + U(rng);
   r.flags = (rng() % 4 == 0 ? RADIS_LDM_GENERIC_FFT : 0) | (rng() % 5 == 0 ? RADIS_LDM_PERIOD_L : 0) |
@@ -104,13 +108,15 @@ struct HostBatch {
 };
 
 char g_work_base[1];
-int g_splits = 0;  // requests laid out with a split pass B  // job tables hold offsets from a workspace base; never dereferenced here
+int g_splits = 0;  // requests laid out with a split pass B
+int g_grouped = 0;  // requests laid out with fill row groups  // job tables hold offsets from a workspace base; never dereferenced here
 
 void layout(const radis_ldm_db& db, HostBatch& B, int n_cu) {
   B.total = 0;
   choose_split_b(B.reqs.data(), (int)B.reqs.size(), n_cu);
   for (Req& R : B.reqs) {
     g_splits += R.split_b == 2;
+    g_grouped += R.fill_groups > 1;
     const int rc = layout_request(R, B.total);
     CHECK(rc == 0, "layout_request: %s", g_err);
   }
@@ -133,7 +139,7 @@ void layout(const radis_ldm_db& db, HostBatch& B, int n_cu) {
       } else if (i < nax + P.n_wrap) {
         CHECK(2 * t.y < R.wrap_rows, "wrap pair beyond the table");
       } else if (i < nax + P.n_wrap + P.n_fill) {
-        CHECK(t.y >= 0 && t.y < R.n_tiles, "tile");
+        CHECK(t.y >= 0 && t.y < R.n_tiles * R.fill_groups, "tile / row group");
       } else if (i >= nax + P.n_wrap + P.n_fill + P.pcap * P.n_colA && i < n - P.n_colC) {
         // pass B: (row block, share), each once, in order
         const int k = i - (nax + P.n_wrap + P.n_fill + P.pcap * P.n_colA);
@@ -164,7 +170,10 @@ void layout(const radis_ldm_db& db, HostBatch& B, int n_cu) {
                                  R.plan.L > 2 * R.nv),
           "wrap bins %d, n1 %d, compiled %d", R.plan.wrap_k, R.plan.n1, (int)R.plan.fc);
     CHECK((R.plan.wrap_k == 0) == (R.plan.L == 2 * R.nv || (R.r.flags & RADIS_LDM_PERIOD_L)), "wrap iff L != 2 N_v");
-    CHECK(fill_tile_bytes(R.m_cap, R.tile_w) <= (size_t)FILL_LDS_MAX, "fill tile LDS");
+    CHECK(fill_tile_bytes(R.fill_rows, R.tile_w) <= (size_t)FILL_LDS_MAX, "fill tile LDS");
+    CHECK(R.fill_groups >= 1 && (long long)R.fill_rows * R.fill_groups >= R.m_cap &&
+              (R.fill_groups == 1 ? R.fill_rows == R.m_cap : R.fill_rows % 2 == 0),
+          "fill row groups: %d x %d rows for %d", R.fill_groups, R.fill_rows, R.m_cap);
   }
   // the job table: exactly one LdmJob per request
   B.jobs.assign(B.reqs.size(), LdmJob{});
@@ -259,8 +268,9 @@ int main() {
           "nv %lld: L %lld wrap %d", (long long)nv, (long long)p.L, p.wrap_k);
   }
   CHECK(g_splits > 0, "no request took the split pass B");
-  std::printf("planned %d requests, %d refused, %d re-plans, %d split pass B layouts, %d failed checks\n", planned,
-              refused, replans, g_splits, g_fail);
+  CHECK(g_grouped > 0, "no request took fill row groups");
+  std::printf("planned %d requests, %d refused, %d re-plans, %d split pass B layouts, %d row-group layouts, "
+              "%d failed checks\n", planned, refused, replans, g_splits, g_grouped, g_fail);
   if (g_fail) return 1;
   std::printf("ok\n");
   return 0;
