@@ -169,6 +169,7 @@ void db_release(radis_ldm_db* db) {
 struct LaunchGroup {
   int cls, first, count;
   int rows;  // pass B: the rows' coefficients staged in LDS (the largest m_cap of the group, at most FC_B_ROWS_LDS)
+  int wh;    // pass B: L-axis nodes of the image-correction values staged in LDS (largest wrap_rows, 0: none)
 };
 
 struct radis_ldm_batch {
@@ -296,7 +297,7 @@ int build_layout(radis_ldm_batch* b) {
   // the order plan_item gives them
   std::vector<int4> ax, fi, wr;
   std::map<int, std::vector<int4>> ma, mb, mc;
-  std::map<int, int> mb_rows;  // pass B class -> largest m_cap
+  std::map<int, int> mb_rows, mb_wh;  // pass B class -> largest m_cap, largest wrap_rows
   for (int j = 0; j < b->n; ++j) {
     Req& R = b->reqs[j];
     const LdmItemPlan P = item_plan(*b->db, R);
@@ -317,6 +318,7 @@ int build_layout(radis_ldm_batch* b) {
     put(ma[ca_], ba, bb);
     put(mb[cb_], bb, bc);
     mb_rows[cb_] = std::max(mb_rows[cb_], R.m_cap);
+    mb_wh[cb_] = std::max(mb_wh[cb_], R.wrap_rows);
     put(mc[ca_ | (R.split_b > 1 ? FC_C_SUM : 0)], bc, be);  // split pass B: pass C adds two shares
   }
   std::vector<int4> fa, fb, fc;
@@ -324,13 +326,16 @@ int build_layout(radis_ldm_batch* b) {
                     std::vector<radis_ldm_batch::Group>& g) {
     g.clear();
     for (auto& kv : m) {
-      g.push_back({kv.first, (int)v.size(), (int)kv.second.size(), 0});
+      g.push_back({kv.first, (int)v.size(), (int)kv.second.size(), 0, 0});
       v.insert(v.end(), kv.second.begin(), kv.second.end());
     }
   };
   flatten(ma, fa, b->grp_a);
   flatten(mb, fb, b->grp_b);
-  for (auto& G : b->grp_b) G.rows = std::min(mb_rows[G.cls], FC_B_ROWS_LDS);
+  for (auto& G : b->grp_b) {
+    G.rows = std::min(mb_rows[G.cls], FC_B_ROWS_LDS);
+    G.wh = mb_wh[G.cls];
+  }
   flatten(mc, fc, b->grp_c);
   b->n_axis = (int)ax.size();
   b->n_wrap = (int)wr.size();
@@ -487,8 +492,9 @@ int launch_set(const LaunchSet& X, const LdmDb& dbd, radis_ldm_batch* b, hipStre
     const dim3 grid(G.count);
 #define FC_B(N) \
   case N:                                                                                                    \
-    launch_part(b, timed, slot, 5, f, l, k_fft_b_c<N>, grid, dim3(FC_T_B), (size_t)G.rows * sizeof(float4), s, cj, \
-                ca, it, G.rows);                                                                               \
+    launch_part(b, timed, slot, 5, f, l, k_fft_b_c<N>, grid, dim3(FC_T_B),                                   \
+                (size_t)G.rows * sizeof(float4) + (size_t)2 * fc_rows_b(N) * G.wh * sizeof(float), s, cj, ca, it,  \
+                G.rows, G.wh);                                                                                 \
     break;
     switch (G.cls) {
       FC_B(64) FC_B(128) FC_B(256) FC_B(512) FC_B(1024) FC_B(2048)
@@ -1162,8 +1168,9 @@ int radis_ldm_run_noalloc(radis_ldm_ctx* ctx, const radis_ldm_db* db, const radi
   HIPCHK(hipGetLastError());
   const int cls_ac = R.plan.fc ? R.plan.n1 : 0;
   const int cls_b = R.plan.fc ? R.plan.n2 : (R.plan.n2 <= LDM_CAP_B ? 0 : -1);
-  const LaunchGroup ga{cls_ac, 0, P.pcap * P.n_colA, 0}, gb{cls_b, 0, P.n_rowB, std::min(R.m_cap, FC_B_ROWS_LDS)},
-      gc{cls_ac | (R.split_b > 1 ? FC_C_SUM : 0), 0, P.n_colC, 0};
+  const LaunchGroup ga{cls_ac, 0, P.pcap * P.n_colA, 0, 0},
+      gb{cls_b, 0, P.n_rowB, std::min(R.m_cap, FC_B_ROWS_LDS), R.wrap_rows},
+      gc{cls_ac | (R.split_b > 1 ? FC_C_SUM : 0), 0, P.n_colC, 0, 0};
   LaunchSet X{};
   X.jobs = (LdmJob*)(base + o_job);
   X.axes = (LdmAxes*)(base + o_axes);

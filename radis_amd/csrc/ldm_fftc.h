@@ -414,13 +414,15 @@ __global__ __launch_bounds__(FC_T_A) void k_fft_a_c(const LdmJob* __restrict__ j
 // spectrum goes through LDS into the inverse transform, whose last stage
 // writes B with the W_L twiddle.  Item (job, row block, share): with
 // J.bbuf2 set, share 0 takes the first ceil(P/2) pairs into B and share 1
-// the rest into bbuf2 (pass C adds the two).  The rows' lineshape
-// coefficients go to the dynamic LDS (n_rows_lds float4, launch-sized).
+// the rest into bbuf2 (pass C adds the two).  The dynamic LDS (sized per
+// launch) holds the rows' lineshape coefficients (n_rows_lds float4), then
+// the image-correction values of the block's rows at its two corrected bins
+// (2 x B x n_wh floats, n_wh >= N_L of every request with a correction).
 // ---------------------------------------------------------------------------
 template <int N2>
 __global__ __launch_bounds__(FC_T_B, (N2 >= 1024 || N2 == 256) ? 4 : 2) void k_fft_b_c(const LdmJob* __restrict__ jobs,
                                                       const LdmAxes* __restrict__ axes,
-                                                      const int4* __restrict__ items, int n_rows_lds) {
+                                                      const int4* __restrict__ items, int n_rows_lds, int n_wh) {
   constexpr int CAP = FC_CAP_B, T = FC_T_B;
   constexpr int B = fc_rows_b(N2), LB = fc_log2(B);
   static_assert(fc_pow2(N2), "pass B rows are powers of two");
@@ -525,7 +527,8 @@ __global__ __launch_bounds__(FC_T_B, (N2 >= 1024 || N2 == 256) ? 4 : 2) void k_f
   // are staged in LDS here, so the pair loop reads no global memory for
   // them (a global load there would wait behind the next pair's prefetch).
   // w_lo / w_hi hold the row b of those bins (-1: none), w_s the second's s.
-  __shared__ float s_wh[2][B][LDM_MAX_AX];
+  float* const s_wh = reinterpret_cast<float*>(s_row + n_rows_lds);  // [2][B][n_wh]
+  auto wh = [&](int c, int b, int m) -> float& { return s_wh[(c * B + b) * n_wh + m]; };
   const int WK = J.wrap_k;
   int w_lo = -1, w_hi = -1, w_s = 0;
   if (WK) {
@@ -533,7 +536,7 @@ __global__ __launch_bounds__(FC_T_B, (N2 >= 1024 || N2 == 256) ? 4 : 2) void k_f
     for (int i = threadIdx.x; i < 2 * B * NLw; i += T) {
       const int c = i / (B * NLw), b = (i / NLw) % B, m = i % NLw;
       const int k1 = k1_0 + b, kk = c ? N1 - k1 : k1;
-      s_wh[c][b][m] = kk < WK ? J.wrap_h[m * WK + kk] : 0.0f;
+      wh(c, b, m) = kk < WK ? J.wrap_h[m * WK + kk] : 0.0f;
     }
     if ((int)threadIdx.x < B && k1_0 + (int)threadIdx.x < WK) w_lo = threadIdx.x;
 #pragma unroll
@@ -585,9 +588,9 @@ __global__ __launch_bounds__(FC_T_B, (N2 >= 1024 || N2 == 256) ? 4 : 2) void k_f
     }
     if (w_lo >= 0 || w_hi >= 0) {
       const int m = r % NLr;
-      if (w_lo >= 0) F[0][0] += s_wh[0][w_lo][m];
+      if (w_lo >= 0) F[0][0] += wh(0, w_lo, m);
       if (w_hi >= 0) {
-        const float hv = s_wh[1][w_hi][m];
+        const float hv = wh(1, w_hi, m);
 #pragma unroll
         for (int s = 0; s < SH::ITL; ++s)
           if (s == w_s) F[s][SH::RL - 1] += hv;

@@ -10,7 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define LDM_MAX_AX 64      // maximum N_G or N_L
+#define LDM_MAX_AX 128     // maximum N_G or N_L
 #define LDM_MAX_ISO 16     // isotopologue slots
 #define LDM_MAX_RAD 12     // radix stages per sub-FFT
 #define LDM_CAP 8192       // complex elements held in LDS by a pass A/C workgroup (and a large pass B one)

@@ -547,7 +547,7 @@ def read_result(result):
 def describe_status(st):
     parts = []
     if st & _native.ST_AXIS:
-        parts.append("LDM axis longer than 64 nodes")
+        parts.append("LDM axis longer than 128 nodes")
     if st & _native.ST_CAP:
         parts.append("LDM rows above the planned capacity")
     if st & _native.ST_FOLD:

@@ -95,3 +95,27 @@ def test_row_group_launch_with_an_ordinary_request(dev):
         assert err <= 1e-4
     b.close()
     db.close()
+
+
+def test_lorentzian_axis_beyond_64_nodes(dev):
+    """N_L between 64 and 128 (dxL 0.015): the axis tables, the image
+    correction of every L node (pass B stages it for the block's rows in
+    launch-sized LDS) and the spectrum against the oracle."""
+    lines = _lines(20_000, 2000.5, 2199.5, seed=73)
+    c = workloads.Conditions(2000.0, 2200.0, 0.01, 1000.0, 2.0, 0.3, dxL=0.015)
+    db = engine.LineDatabase(dev, lines)
+    b = engine.SpectrumBatch(db, [engine.request_from_conditions(db, c)])
+    out = b.results(b.run())[0]
+    inf = b.info(0)
+    ref, det = _oracle(lines, c)
+    print(f"N_G={det['NG']} N_L={det['NL']} L={inf.fft_len} 2N_v={2 * inf.n_v}", flush=True)
+    assert 64 < det["NL"] <= 128 and (inf.n_g, inf.n_l) == (det["NG"], det["NL"])
+    assert inf.fft_len != 2 * inf.n_v  # the image correction runs
+    t = b.line_terms(0)
+    np.testing.assert_array_equal(t["l0"], det["li0"])
+    np.testing.assert_array_equal(t["m0"], det["mi0"])
+    err = oracle.rel_peak_error(out, ref)
+    print(f"N_L > 64: rel err {err:.3e}", flush=True)
+    assert err <= 1e-4
+    b.close()
+    db.close()
