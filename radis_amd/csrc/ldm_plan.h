@@ -38,6 +38,10 @@ const int FILL_LDS_MAX = 60 * 1024;      // hard limit (plus ~2.2 KB of static a
 const int MIN_TILE_W = 16, MAX_TILE_W = 4096;
 const int FILL_GROUP_TILE_W = 32;      // tile width of fills in row groups
 const int N1_MAX = 1024, N2_MAX = 8192;
+// grids beyond N1_MAX * N2_MAX: run-time-sized passes A and C with n1 up to
+// the LDS capacity (one to a few columns per workgroup: slow, but any grid
+// up to N1_BIG * N2_MAX = 2^26 points transforms)
+const int N1_BIG = LDM_CAP;
 
 // last error of the calling thread (a fixed buffer: reporting an error
 // allocates nothing)
@@ -384,14 +388,16 @@ int64_t wrap_floor(int64_t need) {
   return L;
 }
 
-// Search L = n1*n2 >= 2*N_v (or == want) with 16 | n1 <= 1024 and
-// 16 | n2 <= n2_max, 7-smooth cofactors; smallest L, then n1 closest to 512.
-int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int only_n1 = 0) {
+// Search L = n1*n2 >= 2*N_v (or == want) with 16 | n1 <= 1024 (n1_big:
+// <= N1_BIG) and 16 | n2 <= n2_max, 7-smooth cofactors; smallest L, then n1
+// closest to 512.
+int64_t search_fft(int64_t need, int64_t want, int n2_max, int& b1, int& b2, int only_n1 = 0, bool n1_big = false) {
   static const SmoothList c1(N1_MAX);
+  static const SmoothList c1b(N1_BIG);
   static const SmoothList c2(N2_MAX);
   int64_t bestL = -1;
   double bestScore = 1e300;
-  for (int n1 : c1) {
+  for (int n1 : n1_big ? c1b : c1) {
     if (only_n1 && n1 != only_n1) continue;
     int n2 = 0;
     if (want > 0) {
@@ -438,14 +444,19 @@ int plan_fft(int64_t nv, int64_t want, FftPlan& out, int only_n1 = 0, bool allow
     return search_fft(lo, 0, n2_max, b1, b2, only_n1);
   };
   int a1 = 0, a2 = 0, s1 = 0, s2 = 0;
-  const int64_t any = pick(N2_MAX, a1, a2);
+  int64_t any = pick(N2_MAX, a1, a2);
   const int64_t small = pick(LDM_CAP_B, s1, s2);  // pass B with many small tiles
+  if (any < 0 && !only_n1 && want == 0) {
+    // beyond N1_MAX * N2_MAX: a run-time-sized split with a longer n1
+    const int64_t exact = search_fft(need, need, N2_MAX, a1, a2, 0, true);
+    any = exact > 0 ? exact : search_fft(lo, 0, N2_MAX, a1, a2, 0, true);
+  }
   if (any < 0) {
     if (want > 0)
       return fail(RADIS_LDM_ERANGE, "fft_len %lld is not n1*n2 with 16|n1<=%d, 16|n2<=%d and 7-smooth cofactors",
                   (long long)want, N1_MAX, N2_MAX);
     return fail(RADIS_LDM_ERANGE, "grid of %lld points needs an FFT above %lld points", (long long)nv,
-                (long long)N1_MAX * N2_MAX);
+                (long long)N1_BIG * N2_MAX);
   }
   const bool use_small = small > 0 && (double)small <= 1.03 * (double)any;
   out.L = use_small ? small : any;

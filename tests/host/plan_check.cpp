@@ -259,14 +259,23 @@ int main() {
     }
   }
   // planner rules on their own: the transform length is 2 N_v or leaves the
-  // image correction its gap, for every grid length up to 2^20
-  for (int64_t nv = 2; nv < (1 << 20); nv = nv * 9 / 8 + 1) {
+  // image correction its gap, for every grid length up to 2^25 (beyond
+  // 1024 x 8192 points the split takes a longer n1, up to the LDS capacity)
+  int big = 0;
+  for (int64_t nv = 2; nv < (1 << 25); nv = nv * 9 / 8 + 1) {
     FftPlan p;
     const int rc = plan_fft(nv, 0, p);
-    if (rc) continue;  // grids too short for the correction report an error
+    if (rc) {
+      CHECK(nv < 1000 || wrap_floor(2 * nv) > (int64_t)N1_BIG * N2_MAX, "nv %lld refused: %s", (long long)nv, g_err);
+      continue;  // grids too short for the correction, or beyond the longest split, report an error
+    }
     CHECK(p.L == 2 * nv || (p.wrap_k > 0 && p.wrap_k <= LDM_WRAP_KMAX && p.L >= wrap_floor(2 * nv)),
           "nv %lld: L %lld wrap %d", (long long)nv, (long long)p.L, p.wrap_k);
+    CHECK(p.n1 <= (p.L > (int64_t)N1_MAX * N2_MAX ? N1_BIG : N1_MAX) && p.n2 <= N2_MAX && p.L == (int64_t)p.n1 * p.n2,
+          "nv %lld: split %d x %d", (long long)nv, p.n1, p.n2);
+    big += p.n1 > N1_MAX;
   }
+  CHECK(big > 0, "no grid took a split with n1 above %d", N1_MAX);
   CHECK(g_splits > 0, "no request took the split pass B");
   CHECK(g_grouped > 0, "no request took fill row groups");
   std::printf("planned %d requests, %d refused, %d re-plans, %d split pass B layouts, %d row-group layouts, "

@@ -106,3 +106,34 @@ def test_grid_filling_half_the_transform(dev, n1):
     assert oracle.rel_peak_error(out, ref) <= 1e-4
     b.close()
     db.close()
+
+
+@pytest.mark.gpu
+def test_grid_beyond_the_compiled_and_short_splits(dev):
+    """A 6,000,001-point grid (2 N_v = 12 M, beyond 1024 x 8192): the planner
+    takes a run-time-sized split with n1 above 1024 (passes A and C hold one
+    or a few columns per workgroup); against the oracle at L = 2 N_v."""
+    rng = np.random.default_rng(81)
+    n = 2000
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    lines = workloads.LineList(
+        nu0=np.sort(rng.uniform(2001.0, 2599.0, n)), S_ref=f32(10.0 ** rng.uniform(-22.0, -20.0, n)),
+        El=f32(rng.uniform(0.0, 2000.0, n)), g_air=f32(rng.uniform(0.050, 0.051, n)),
+        g_self=f32(np.full(n, 0.07)), n_air=f32(np.full(n, 0.7)), molar_mass=np.array([0.0, 28.0]),
+        q_coef=(0.94, 1.1), name="long_grid")
+    wmax = 2000.0 + 6_000_000 * 1e-4
+    db = engine.LineDatabase(dev, lines)
+    cond = dict(wmin=2000.0, wmax=wmax, wstep=1e-4, Tgas=500.0, pressure_bar=0.2, mole_fraction=0.1)
+    b = engine.SpectrumBatch(db, [engine.make_request(db, **cond)])
+    out = b.results(b.run())[0]
+    inf = b.info(0)
+    print(f"N_v={inf.n_v} L={inf.fft_len} split {inf.n1} x {inf.n2} compiled={inf.compiled_fft} "
+          f"N_G={inf.n_g} N_L={inf.n_l}", flush=True)
+    assert 2 * inf.n_v > 1024 * 8192 and inf.n1 > 1024 and inf.compiled_fft == 0
+    ref = oracle.eq_abscoeff(lines.as_dict(), cond["wmin"], wmax, 1e-4, cond["Tgas"], cond["pressure_bar"],
+                             cond["mole_fraction"], workloads.DXG_DEFAULT, workloads.DXL_DEFAULT)
+    err = oracle.rel_peak_error(out, ref)
+    print(f"long grid: rel err {err:.3e}", flush=True)
+    assert err <= 1e-4
+    b.close()
+    db.close()
