@@ -58,6 +58,15 @@ class Spectrum:
         from . import slit as _slit
 
         w = self._w
+        if isinstance(slit_function, (int, float)) and float(slit_function) == 0.0:
+            # no slit (the interactive view's default): the convolved
+            # quantities are the unconvolved ones
+            if store:
+                for qns in [k for k in self._q if k.endswith("_noslit")]:
+                    self._q[qns[: -len("_noslit")]] = self._q[qns].copy()
+                self._slit = (np.array([0.0]), np.array([1.0]))
+                self.conditions.update(slit_function=0.0, slit_unit=unit, slit_shape=shape, norm_by=norm_by)
+            return self
         wstep = float(np.abs(np.diff(w)).min())
         if center_wavespace is None:
             center_wavespace = w[len(w) // 2]
@@ -110,8 +119,31 @@ class Spectrum:
         NwL, NwG = self.gpu_app.get_griddims()
         self.conditions["NwL"], self.conditions["NwG"] = NwL, NwG
         self._set_quantities(abscoeff, self.gpu_app.derived)
+        if getattr(self, "_slit", None) is not None:
+            # the slit-convolved quantities follow the new conditions (RADIS
+            # recomputes them from the conditions when they are asked for)
+            c = self.conditions
+            self.apply_slit(c.get("slit_function", 0.0), unit=c.get("slit_unit", "cm-1"),
+                            shape=c.get("slit_shape", "triangular"), norm_by=c.get("norm_by", "area"))
         self.conditions["calculation_time"] = time.perf_counter() - t0
         return self.get(var, wunit=wunit, Iunit=Iunit, copy=False)[1]
+
+    def plot(self, var=None, wunit="default", Iunit="default", show=False, **kwargs):
+        """Plot ``var`` against wavenumber with matplotlib; returns the
+        Line2D (radis/spectrum/spectrum.py plot, cm-1 and default units)."""
+        import matplotlib.pyplot as plt
+
+        var = var or self.get_vars()[0]
+        w, y = self.get(var, wunit=wunit, Iunit=Iunit)
+        fig = plt.figure()
+        ax = fig.add_subplot(111)
+        (line,) = ax.plot(w, y, **kwargs)
+        ax.set_xlabel("Wavenumber (cm-1)")
+        unit = self.UNITS.get(var, "")
+        ax.set_ylabel(f"{var} ({unit})" if unit else var)
+        if show:
+            plt.show()
+        return line
 
     def exit_gpu(self, event=None):
         """radis/spectrum/spectrum.py:6574-6577."""
@@ -210,8 +242,62 @@ class SpectrumFactory:
             app.__del__()
         return s
 
-    def eq_spectrum_gpu_interactive(self, *args, **kwargs):
-        raise NotImplementedError("matplotlib sliders are outside this package; use Spectrum.recalc_gpu")
+    def eq_spectrum_gpu_interactive(self, var="transmittance", slit_function=0.0, mpl_backend="", plotkwargs={},
+                                    *vargs, **kwargs):
+        """``eq_spectrum_gpu`` with sliders for every condition given as a
+        ``plot_tools.ParamRange`` (and for ``slit_function``): each slider
+        move writes the condition and recomputes the plotted ``var`` on the
+        resident GPU application (``Spectrum.recalc_gpu``); closing the
+        figure releases it.  radis/lbl/factory.py:1393-1523."""
+        from matplotlib import use
+
+        from .plot_tools import is_range
+
+        if mpl_backend:
+            use(mpl_backend)
+        import matplotlib.pyplot as plt
+        from matplotlib.widgets import Slider
+
+        self.interactive_params = {}
+        for key in kwargs:
+            if is_range(kwargs[key]):
+                param_range = kwargs[key]
+                self.interactive_params[key] = param_range
+                param_range.name = key
+                kwargs[key] = param_range.valinit
+        kwargs["exit_gpu"] = False
+        s = self.eq_spectrum_gpu(*vargs, **kwargs)
+        if is_range(slit_function):
+            self.interactive_params["slit_function"] = slit_function
+            slit_function.name = "slit_function"
+            slit_function = slit_function.valinit
+        s.apply_slit(slit_function, unit="cm-1")  # the slit quantities: 'radiance', 'transmittance'
+        s.conditions["slit_function"] = slit_function
+
+        was_interactive = plt.isinteractive()
+        plt.ion()
+        line = s.plot(var, show=True, **plotkwargs)
+        fig = line.figure
+
+        def update_plot(val):
+            # the sliders write s.conditions; recalc_gpu reads them
+            new_y = s.recalc_gpu(var, wunit=plotkwargs.get("wunit", "default"),
+                                 Iunit=plotkwargs.get("Iunit", "default"))
+            line.set_ydata(new_y)
+            fig.canvas.draw_idle()
+
+        n_sliders = 0
+        for key, param_range in self.interactive_params.items():
+            slider_axis = fig.add_axes([0.25, 0.05 * n_sliders + 0.05, 0.65, 0.03])
+            slider = Slider(ax=slider_axis, label=key, valmin=param_range.valmin, valmax=param_range.valmax,
+                            valinit=param_range.valinit)
+            param_range.set_widget(slider, s, update_plot)
+            n_sliders += 1
+        fig.subplots_adjust(bottom=0.05 * n_sliders + 0.15)
+        fig.canvas.mpl_connect("close_event", s.exit_gpu)
+        if not was_interactive:
+            plt.ioff()
+        return s
 
 
 class _LineApp:

@@ -248,3 +248,30 @@ def test_apply_slit_on_gpu_matches_oracle():
         assert np.isclose(np.trapezoid(r[ok], w[ok]), np.trapezoid(r0[ok], w[ok]), rtol=3e-2)
     with pytest.raises(AssertionError):
         s.apply_slit(500.0)  # wider than the spectral range in 'valid' mode
+
+
+def test_eq_spectrum_gpu_interactive_sliders():
+    """radis/lbl/factory.py:1393-1523 (docstring example): sliders for Tgas,
+    path_length and the slit; moving one recomputes the plotted quantity on
+    the resident GPU application, equal to a fresh computation at the new
+    conditions."""
+    from radis_amd.plot_tools import ParamRange
+
+    lines = hitran.line_list(HITRAN)
+    sf = factory.SpectrumFactory(wavenum_min=2100.0, wavenum_max=2200.0, wstep=0.002, lines=lines)
+    s = sf.eq_spectrum_gpu_interactive(var="transmittance", Tgas=ParamRange(300.0, 2000.0, 1200.0),
+                                       pressure=0.2, mole_fraction=0.1, path_length=ParamRange(0, 10, 2.0),
+                                       slit_function=ParamRange(0, 1, 0), mpl_backend="Agg")
+    assert set(sf.interactive_params) == {"Tgas", "path_length", "slit_function"}
+    line = sf.interactive_params["Tgas"].widget.ax.figure.axes[0].lines[0]  # the plotted quantity
+    sf.interactive_params["Tgas"].widget.set_val(1500.0)
+    sf.interactive_params["path_length"].widget.set_val(5.0)
+    assert s.conditions["Tgas"] == 1500.0 and s.conditions["path_length"] == 5.0
+    ref = sf.eq_spectrum_gpu(Tgas=1500.0, pressure=0.2, mole_fraction=0.1, path_length=5.0, exit_gpu=True)
+    np.testing.assert_allclose(s.get("transmittance")[1], ref.get("transmittance_noslit")[1], atol=1e-6)
+    sf.interactive_params["slit_function"].widget.set_val(0.5)
+    ref.apply_slit(0.5, unit="cm-1")
+    np.testing.assert_allclose(s.get("transmittance")[1], ref.get("transmittance")[1], atol=1e-6)
+    np.testing.assert_array_equal(line.get_ydata(), s.get("transmittance")[1])
+    s.exit_gpu()
+    assert s.gpu_app is None
