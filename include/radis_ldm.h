@@ -38,7 +38,8 @@ enum {
     RADIS_LDM_EINVAL = -1,   /* invalid argument (shape, range, unsorted nu0 ...) */
     RADIS_LDM_EHIP = -2,     /* HIP runtime error */
     RADIS_LDM_ENOMEM = -3,   /* device allocation failed or exceeds the byte budget */
-    RADIS_LDM_ERANGE = -4,   /* grid or LDM exceeds a compiled limit */
+    RADIS_LDM_ERANGE = -4,   /* grid or LDM exceeds a compiled limit (LDM axes of more than 128
+                                nodes, transforms above 8192 x 8192 points, 2^29 lines) */
     RADIS_LDM_EDEVICE = -5,  /* device-side check failed (see radis_ldm_batch_status) */
 };
 
