@@ -18,8 +18,15 @@ replay, every one checked against the CPU oracle at RADIS's FFT length) and
   --workload serve                          the served form alone
   --workload serve_sweep                    serving capacity: offered load
                                             from 200 req/s to saturation
+  --window-inset CM                         with a single-spectrum workload:
+                                            the window narrowed by CM cm-1 at
+                                            both ends, strictly inside the
+                                            list (the per-block axis reduction
+                                            runs instead of the whole-list
+                                            Pareto sets)
 """
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -49,6 +56,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="tune the fill tile width and the FFT schedule per size class before timing "
                          "(cache in a fresh temporary file)")
+    ap.add_argument("--window-inset", type=float, default=0.0, metavar="CM",
+                    help="narrow a single-spectrum workload's window by CM cm-1 at both ends")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -233,6 +242,8 @@ def bench_single(args, workload):
 
     log(f"{workload}: generating lines")
     lines, cond = workloads.workload(workload, n=args.lines or None)
+    if args.window_inset:
+        cond = dataclasses.replace(cond, wmin=cond.wmin + args.window_inset, wmax=cond.wmax - args.window_inset)
     n_lines = len(lines)
     dev = engine.Device(0)
     log(f"{workload}: uploading {n_lines} lines")
@@ -339,6 +350,13 @@ def bench_single(args, workload):
             "p_bar": cond.pressure_bar,
             "x": cond.mole_fraction,
             "wstep": cond.wstep,
+            "window": [cond.wmin, cond.wmax],
+            # a window holding every line of a single-isotopologue list takes the
+            # list's Pareto sets in k_finalize; any other runs k_axis over the
+            # window's blocks
+            "axis_reduction": "whole-list Pareto sets" if n_win == n_lines else "k_axis over the window's blocks",
+            "axis_candidates": int(inf.axis_candidates),
+            "axis_scanned_lines": int(inf.axis_scanned),
             "graph": bool(args.graph),
             "compiled_fft": bool(inf.compiled_fft),
             "tile_w": int(inf.tile_w),
