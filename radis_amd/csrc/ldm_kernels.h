@@ -130,8 +130,11 @@ __device__ __forceinline__ double fwhm_gauss(double nu0, double dopp) {
 
 // Lorentzian FWHM, radis/lbl/broadening.py:341-383 (x2 at 1854), with
 // diluent {'air': 1-x}; (Tref/T)**n as pow, as NumPy evaluates it (the
-// width extremes the LDM axes start from).
-__device__ __forceinline__ double fwhm_lorentz(const LdmJob& J, double g_air, double g_self, double n_air,
+// width extremes the LDM axes start from).  A called function: k_axis and
+// k_finalize evaluate it at several sites, and one copy of its two FP64 pow
+// routines keeps their code short (short kernels that fetch their code cold
+// once per spectrum).
+__device__ __noinline__ double fwhm_lorentz(const LdmJob& J, double g_air, double g_self, double n_air,
                                                double n_self) {
 #pragma clang fp contract(off)
   const double ra = pow(J.ratio_T, n_air);
@@ -307,12 +310,18 @@ __global__ __launch_bounds__(256) void k_qspline_eval(const LdmQSpline* __restri
 }
 
 // Voigt FT at x for FWHM (wG, wL): exp(-(pi x wG)^2/(4 ln2)) * exp(-pi x wL)
-// (radis/lbl/broadening.py:252, 604, 850-853 with hwhm = fwhm/2).
-__device__ __forceinline__ double voigt_ft64(double x, double wG, double wL) {
+// (radis/lbl/broadening.py:252, 604, 850-853 with hwhm = fwhm/2); a called
+// function, one copy for k_finalize's two sites.
+__device__ __noinline__ double voigt_ft64(double x, double wG, double wL) {
   const double PI = 3.141592653589793;
   const double g = PI * x * wG;
   return exp(-(g * g) / (4.0 * 0.6931471805599453)) * exp(-PI * x * wL);
 }
+
+// FP64 exp and log as called functions in k_finalize (one copy each of
+// routines it runs at several sites; same routines, same results)
+__device__ __noinline__ double exp_call(double x) { return exp(x); }
+__device__ __noinline__ double log_call(double x) { return log(x); }
 
 // ---------------------------------------------------------------------------
 // k_finalize: one workgroup per request.  Builds the two log-spaced axes
@@ -415,8 +424,8 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
           status |= LDM_ST_WIDTH;
         } else {
           // N = ceil((log(w_max + 1e-4) - log(w_min)) / dx) + 1, radis/lbl/broadening.py:1835-1836
-          ng = (int)ceil((log(wg_max + 1e-4) - log(wg_min)) / dxG) + 1;
-          nl = (int)ceil((log(wl_max + 1e-4) - log(wl_min)) / dxL) + 1;
+          ng = (int)ceil((log_call(wg_max + 1e-4) - log_call(wg_min)) / dxG) + 1;
+          nl = (int)ceil((log_call(wl_max + 1e-4) - log_call(wl_min)) / dxL) + 1;
           if (ng > LDM_MAX_AX || nl > LDM_MAX_AX || ng < 2 || nl < 2) status |= LDM_ST_AXIS;
           else if (ng * nl > m_cap) status |= LDM_ST_CAP;
         }
@@ -434,13 +443,13 @@ __global__ __launch_bounds__(256) void k_finalize(LdmJob* __restrict__ jobs, Ldm
   }
   // axis nodes w_min * exp(dx * i) and their logs, radis/lbl/broadening.py:1837
   for (int i = threadIdx.x; i < s_ng; i += blockDim.x) {
-    wgax[i] = s_wg_min * exp(dxG * (double)i);
-    lgax[i] = log(wgax[i]);
+    wgax[i] = s_wg_min * exp_call(dxG * (double)i);
+    lgax[i] = log_call(wgax[i]);
     A.laG[i] = lgax[i];
   }
   for (int i = threadIdx.x; i < s_nl; i += blockDim.x) {
-    wlax[i] = s_wl_min * exp(dxL * (double)i);
-    llax[i] = log(wlax[i]);
+    wlax[i] = s_wl_min * exp_call(dxL * (double)i);
+    llax[i] = log_call(wlax[i]);
     A.laL[i] = llax[i];
   }
   __syncthreads();
