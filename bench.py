@@ -351,10 +351,11 @@ def bench_single(args, workload):
             "x": cond.mole_fraction,
             "wstep": cond.wstep,
             "window": [cond.wmin, cond.wmax],
-            # a window holding every line of a single-isotopologue list takes the
-            # list's Pareto sets in k_finalize; any other runs k_axis over the
-            # window's blocks
-            "axis_reduction": "whole-list Pareto sets" if n_win == n_lines else "k_axis over the window's blocks",
+            # a window holding every line of a list without an isotopologue
+            # column takes the list's Pareto sets in k_finalize; any other
+            # runs k_axis over the window's blocks
+            "axis_reduction": ("whole-list Pareto sets" if n_win == n_lines and lines.iso is None
+                               else "k_axis over the window's blocks"),
             "axis_candidates": int(inf.axis_candidates),
             "axis_scanned_lines": int(inf.axis_scanned),
             "graph": bool(args.graph),
